@@ -1,0 +1,206 @@
+/*
+ * sfk.h -- C ABI of the MI355X (gfx950) matrix-free sum-factorization
+ * operators.  Plain C: handles, plain pointers, sizes and status codes; no
+ * HIP, torch or C++ types cross this boundary (a HIP stream is passed as
+ * `void *`, NULL = the default stream).
+ *
+ * Reference interfaces replaced (paths in the reference checkout):
+ *   EvaluationCellLaplacian<dim,degree,Number>   legacy/include/evaluation_cell_laplacian.h:26-908
+ *   EvaluationDGLaplacian<dim,degree,Number>     legacy/include/evaluation_dg_laplacian.h:33-716
+ *   EvaluationDGAdvection<dim,degree,Number>     legacy/include/evaluation_dg_advect.h:96-845
+ *   apply_1d_matvec_kernel<...>                  legacy/include/matrix_vector_kernel.h:30-185
+ * The reference has no FFI: its "plugin API" is those class templates.
+ * The C++ mirror classes in sum_factorization_kernels_amd/include/ keep
+ * their names and methods and call only this ABI.
+ *
+ * Vector layout ("canonical"): a vector of an operator on n_cells[0] x
+ * n_cells[1] x n_cells[2] cells holds cell-major blocks of (p+1)^3 values,
+ * cell = (iz*ny + iy)*nx + ix, dof = (k*(p+1) + j)*(p+1) + i (x fastest).
+ * Continuous Q_p vectors hold the (nx*p+1)(ny*p+1)(nz*p+1) nodes of the
+ * structured grid, x fastest.  Converters to and from the reference's
+ * lane-interleaved layout: sfk_layout_*.
+ *
+ * All device pointers are caller-owned device memory (hipMalloc, torch).
+ * A handle is not thread-safe; use one handle per host thread.  Every
+ * entry point returns an sfk_status; sfk_last_error() returns a
+ * thread-local description of the last failure.
+ */
+#ifndef SFK_H
+#define SFK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SFK_OK = 0,
+  SFK_ERR_INVALID_ARG = 1,  /* bad degree, size, pointer or descriptor field */
+  SFK_ERR_UNSUPPORTED = 2,  /* combination not built into this library */
+  SFK_ERR_HIP = 3,          /* a HIP runtime call failed */
+  SFK_ERR_ALLOC = 4,        /* device allocation failed */
+  SFK_ERR_NONFINITE = 5,    /* non-finite value detected (solver) */
+  SFK_ERR_NOT_READY = 6,    /* required data (metric, coefficient) not set */
+  SFK_ERR_NO_CONVERGENCE = 7
+} sfk_status;
+
+typedef enum {
+  SFK_CELL_LAPLACE = 0, /* element-local Laplacian, EvaluationCellLaplacian */
+  SFK_CONT_LAPLACE = 1, /* continuous Q_p Laplacian, homogeneous Dirichlet  */
+  SFK_DG_SIPG = 2,      /* DG symmetric interior penalty, periodic          */
+  SFK_DG_ADVECT = 3     /* DG upwind advection M^{-1}L(u), periodic         */
+} sfk_op_kind;
+
+typedef enum { SFK_FP64 = 0, SFK_FP32 = 1 } sfk_precision;
+typedef enum { SFK_BASIS_HERMITE = 0, SFK_BASIS_GLL = 1 } sfk_basis;
+typedef enum { SFK_GEOM_CARTESIAN = 0, SFK_GEOM_PER_Q = 1 } sfk_geometry;
+
+/* descriptor flags */
+#define SFK_FLAG_REF_X_SWEEP 1u      /* DG-SIPG: reproduce the reference's last x sweep
+                                        (evaluation_dg_laplacian.h:481-483, S^T) */
+#define SFK_FLAG_VARIABLE_COEF 2u    /* continuous Laplacian: -div(k grad u), k per q-point */
+#define SFK_FLAG_VARIABLE_VELOCITY 4u /* advection: velocity per cell and face q-point */
+
+typedef struct sfk_op_desc {
+  int kind;          /* sfk_op_kind */
+  int degree;        /* polynomial degree p, 1..8 */
+  int precision;     /* sfk_precision: arithmetic and vector element type */
+  int basis;         /* sfk_basis (SFK_CELL_LAPLACE); DG-SIPG is Hermite-like,
+                        advection and continuous Q_p are GLL */
+  int geometry;      /* sfk_geometry */
+  unsigned flags;    /* SFK_FLAG_* */
+  int n_cells[3];    /* cells per direction (SFK_CELL_LAPLACE: {n, 1, 1}) */
+  double inv_jac[3]; /* Cartesian: inverse Jacobian diagonal 1/h_d */
+  double velocity[3];/* advection: constant velocity (without the flag) */
+  int n_rhs;         /* right-hand sides per apply (contiguous vectors), >= 1 */
+} sfk_op_desc;
+
+typedef struct sfk_op sfk_op;
+
+/* Replaces the evaluators' initialize() (evaluation_dg_laplacian.h:43,
+ * evaluation_cell_laplacian.h:34, evaluation_dg_advect.h:107): builds the
+ * 1D tables (fill_shape_values) and the launch plan. */
+sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **op);
+sfk_status sfk_op_destroy(sfk_op *op);
+/* length (elements) of one vector of this operator */
+sfk_status sfk_op_vector_size(const sfk_op *op, long long *n);
+/* number of quadrature points of one cell / all cells, faces per direction */
+sfk_status sfk_op_qdata_sizes(const sfk_op *op, long long *n_cell_q, long long *n_face_q);
+
+/* Geometry / coefficient data (host arrays, copied to device and kept by the
+ * handle; stored at the handle's precision):
+ *  cell metric, SFK_GEOM_PER_Q: 6 values per cell q-point, layout
+ *    [cell][component][q], components (G00,G11,G22,G01,G02,G12) of the
+ *    symmetric G = JxW * J^{-1} J^{-T} (JxW including the quadrature weight);
+ *  face metric, SFK_GEOM_PER_Q (DG-SIPG): for every face direction d and
+ *    every cell c, the face between c and its upper neighbour in d:
+ *    8 values per face q-point, layout [d][cell][component][q], components
+ *    (aL0,aL1,aL2, aU0,aU1,aU2, jxw, sigma) with aX = JxW_f J_X^{-1} n,
+ *    n the unit normal pointing from c to its neighbour, jxw = JxW_f and
+ *    sigma = penalty * JxW_f;
+ *  coefficient (SFK_FLAG_VARIABLE_COEF): k per cell q-point [cell][q];
+ *  velocity (SFK_FLAG_VARIABLE_VELOCITY): cell part [cell][3][q]
+ *    (a_x,a_y,a_z at cell q-points), face part [d][cell][q] = a.n at the
+ *    upper face of cell c in direction d. */
+sfk_status sfk_op_set_cell_metric(sfk_op *op, const double *host, size_t count);
+sfk_status sfk_op_set_face_metric(sfk_op *op, const double *host, size_t count);
+sfk_status sfk_op_set_coefficient(sfk_op *op, const double *host, size_t count);
+sfk_status sfk_op_set_velocity(sfk_op *op, const double *cell_host, size_t cell_count,
+                               const double *face_host, size_t face_count);
+/* Fills the cell (and for DG-SIPG face) metric of the mapping
+ *   x = xhat + amp * sin(pi xhat) sin(pi yhat) sin(pi zhat) * (1,1,1)
+ * of the uniform mesh of [0,1]^3 on the device (setup, not timed).
+ * Penalty: (p+1)^2 * |F| / |K| averaged over both sides. */
+sfk_status sfk_op_set_sine_deformation(sfk_op *op, double amp);
+
+/* dst = A src (n_rhs vectors).  Replaces matrix_vector_product()
+ * (evaluation_cell_laplacian.h:216), do_matvec() (evaluation_dg_laplacian.h
+ * :121, evaluation_dg_advect.h:216). */
+sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream);
+/* dst = A^T src: advection adjoint (downwind); symmetric operators apply A. */
+sfk_status sfk_apply_transpose(sfk_op *op, const void *src, void *dst, void *stream);
+/* Fused Chebyshev update of do_cheby_iter<true> (evaluation_dg_laplacian.h
+ * :485-502, :512-554):  r = A sol_old - rhs;  tmp = c_tm*tmp + c_np*diag*r;
+ * sol_new = sol_old - tmp.  Supported for DG-SIPG and the continuous
+ * Laplacian. */
+sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, const void *diag,
+                          void *sol_new, void *tmp, double c_np, double c_tm, void *stream);
+/* Fused low-storage RK stage of do_rk_stage<true> (evaluation_dg_advect.h
+ * :651-667, :678-715):  r = M^{-1}L(src);  vec_np = upd + c_np*r;
+ * if (c_tm != 0) vec_tm = vec_np + c_tm*r  (vec_tm untouched otherwise, so
+ * it may alias src). */
+sfk_status sfk_rk_stage(sfk_op *op, const void *src, const void *upd, void *vec_np, void *vec_tm,
+                        double c_np, double c_tm, void *stream);
+/* diag = diagonal of A (point-Jacobi); continuous Laplacian: 0 on Dirichlet rows */
+sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream);
+
+/* Gradients (no reference counterpart; north-star extension):
+ * advection: with r = A u - f, J = 0.5 |r|^2, writes dJ/d(velocity) into
+ *   grad_cell [cell][3][q] and grad_face [d][cell][q] (layouts of
+ *   sfk_op_set_velocity); lambda = A u - f is passed in by the caller;
+ * variable-coefficient Laplacian: g_q = -JxW_q grad u_q . grad lambda_q
+ *   summed over the n_rhs pairs, i.e. d(lambda^T A(k) u)/dk_q. */
+sfk_status sfk_grad_velocity(sfk_op *op, const void *u, const void *lambda, void *grad_cell,
+                             void *grad_face, void *stream);
+sfk_status sfk_grad_coefficient(sfk_op *op, const void *u, const void *lambda, void *grad_k,
+                                void *stream);
+
+/* ---- solver (continuous Laplacian) ------------------------------------
+ * Preconditioned CG in FP64 with a Chebyshev(degree) point-Jacobi
+ * preconditioner whose bounds come from a Lanczos estimate of
+ * lambda_max(D^{-1}A) (lanczos_steps FP64 steps, upper bound *= 1.1 (safety), lower =
+ * upper/30).  The preconditioner runs on `smoother_op` (FP32 or FP64 operator
+ * on the same mesh); the residual, inner products, Lanczos and the final
+ * residual stay FP64 on `op` (which must be FP64).  x: initial guess in,
+ * solution out; b: right-hand side (FP64, Dirichlet rows 0). */
+typedef struct sfk_solve_stats {
+  int iterations;
+  double initial_residual, final_residual; /* ||b - A x||_2 (FP64) */
+  double lambda_max, lambda_min;           /* Chebyshev bounds used */
+  double seconds;                          /* wall time of the solve */
+  int matvecs_outer, matvecs_inner;
+} sfk_solve_stats;
+
+sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const double *b, double *x,
+                              double rel_tol, int max_iter, int cheby_degree, int lanczos_steps,
+                              sfk_solve_stats *stats, void *stream);
+
+/* ---- 1D kernel (a standalone batched form of apply_1d_matvec_kernel) ---
+ * Applies the 1D even-odd kernel to n_lines lines of nn values, line l at
+ * in + l*line_stride with element stride `stride` (device FP64 arrays);
+ * dense matrix M[i*nn+q] (row = basis function i, column = point q),
+ * symmetry: M[nn-1-i][nn-1-q] = sym * M[i][q], sym = +1 or -1.
+ * forward: out[q] = sum_i M[i][q] in[i]; backward: out[i] = sum_q M[i][q] in[q]. */
+sfk_status sfk_apply_1d(int nn, const double *matrix_host, int sym, int forward, long long n_lines,
+                        long long line_stride, int stride, const double *in, double *out,
+                        void *stream);
+
+/* ---- 1D tables (fill_shape_values, evaluation_dg_laplacian.h:615-695) ----
+ * Host copies of the tables this library uses, in dense form: S[i*n+q]
+ * basis at Gauss points, D[i*n+q] collocation derivative, Sinv[i*n+g] Gauss
+ * Lagrange at GLL points, w[q] Gauss weights, face[4n] = l_i(0), l_i(1),
+ * l_i'(0), l_i'(1), hermite_d = phi_0'(0). */
+sfk_status sfk_tables(int degree, int basis, double *S, double *D, double *Sinv, double *w,
+                      double *face, double *hermite_d);
+sfk_status sfk_quadrature(int n, double *gauss_points, double *gauss_weights, double *gll_points);
+
+/* ---- layout converters (lane-interleaved VectorizedArray layout of the
+ * reference, evaluation_dg_laplacian.h:162-166) -- device FP64 arrays ---- */
+sfk_status sfk_layout_to_canonical(int degree, const int n_cells[3], int lanes, const double *in,
+                                   double *out, void *stream);
+sfk_status sfk_layout_to_interleaved(int degree, const int n_cells[3], int lanes, const double *in,
+                                     double *out, void *stream);
+
+/* ---- vector helpers on device arrays (FP64 / FP32 per `precision`) ---- */
+sfk_status sfk_vec_fill(int precision, void *x, long long n, double value, void *stream);
+sfk_status sfk_vec_dot(int precision, const void *x, const void *y, long long n, double *result,
+                       void *stream);
+
+const char *sfk_last_error(void);
+const char *sfk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -117,6 +117,23 @@ void sfko_dg_advect_stage(int degree, const int nc[3], const double inv_jac[3],
                           const double vel[3], const double *src, const double *upd,
                           double *vec_np, double *vec_tm, double c_np, double c_tm);
 
+/* ---- extended operators (sfk_oracle_ext.c) ---- */
+void sfko_cont_laplacian(int degree, const int nc[3], const double inv_jac[3], const double *coef,
+                         const double *src, double *dst);
+void sfko_cont_laplacian_coef_grad(int degree, const int nc[3], const double inv_jac[3], int n_rhs,
+                                   const double *u_all, const double *lam_all, double *grad);
+void sfko_dg_sipg_general(int degree, const int nc[3], const double *cell_metric, const double *face_metric,
+                          const double *src, double *dst);
+void sfko_cartesian_metric(int degree, const int nc[3], const double inv_jac[3], double *cell_metric,
+                           double *face_metric);
+void sfko_dg_advect_var(int degree, const int nc[3], const double inv_jac[3], const double *cell_vel,
+                        const double *face_vel, const double *src, double *dst);
+void sfko_dg_mass_inverse(int degree, const int nc[3], const double inv_jac[3], const double *src, double *dst);
+void sfko_dg_advect_var_transpose(int degree, const int nc[3], const double inv_jac[3], const double *cell_vel,
+                                  const double *face_vel, const double *src, double *dst);
+void sfko_dg_advect_velocity_grad(int degree, const int nc[3], const double inv_jac[3], const double *face_vel,
+                                  const double *u, const double *lambda, double *grad_cell, double *grad_face);
+
 /* ---- layout converters (vectorization.h lane interleaving,
  * evaluation_dg_laplacian.h:162-166): interleaved index
  *   ((iz*ny+iy)*(nx/L) + ixb)*L*n^3 + dof*L + lane,  ix = ixb*L + lane. */

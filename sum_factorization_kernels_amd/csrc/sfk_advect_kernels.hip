@@ -1,0 +1,474 @@
+// DG upwind advection on a Cartesian periodic mesh (GLL nodal basis,
+// Gauss quadrature), EvaluationDGAdvection::do_inner_loop / do_rk_stage of
+// legacy/include/evaluation_dg_advect.h:285-715, plus what the reference
+// does not have: per-point velocity, the adjoint (transposed) operator and
+// the gradient with respect to the velocity.
+//
+// kind 0  r = M^{-1} L(u),  L(u)_K = (grad v, a u)_K - <v, (a.n) u_upwind>;
+//         optional fused LSRK stage np = upd + c_np r, tm = np + c_tm r.
+// kind 1  L^T y:  (v, a.grad y)_K - <v, max(a.n,0) (y^- - y^+)>  (downwind
+//         coupling), so that A^T w = L^T (M^{-1} w).
+// kind 2  M^{-1} x = Sinv^T W^{-1} Sinv x (cell-local collocation inverse).
+// kind 3  velocity gradient of mu^T L(a) u with mu = M^{-1} lambda:
+//         cell: (d mu/dx_d) u JxW at every q-point; face (upper face of each
+//         cell, normal velocity b = a.n):  -(mu^- - mu^+) u_upwind JxW_f.
+// Thread/phase organisation as in sfk_cell_kernels.hip.
+#include "sfk_device.h"
+#include "sfk_kernels.h"
+
+namespace sfk {
+
+template <int N>
+constexpr int adv_cells_per_block()
+{
+  return 256 / (N * N) > 0 ? 256 / (N * N) : 1;
+}
+template <int N>
+constexpr int adv_threads()
+{
+  return ((adv_cells_per_block<N>() * N * N + 63) / 64) * 64;
+}
+
+template <int N, typename T>
+struct AdvArgs {
+  Tab<N, T> tab;
+  const T *src, *upd, *aux;
+  T *dst, *tm, *dst2;
+  T c_np, c_tm;
+  const T *cell_vel, *face_vel;
+  T vel[3];
+  T ij[3];
+  T jxw;
+  T jxwf[3];
+  long long ncells;
+  int nx, ny, nz;
+};
+
+// normal velocity a.n (outward from the cell) at face point fa of face (d, s)
+template <int N, typename T, bool VAR>
+__device__ __forceinline__ T normal_velocity(const AdvArgs<N, T> &a, long long cell, int d, int s, int fa)
+{
+  constexpr int N2 = N * N;
+  if constexpr (!VAR) {
+    return s ? a.vel[d] : -a.vel[d];
+  } else {
+    if (s) return a.face_vel[((long long)d * a.ncells + cell) * N2 + fa];
+    const long long nb = neighbor_cell(cell, a.nx, a.ny, a.nz, d, 0);
+    return -a.face_vel[((long long)d * a.ncells + nb) * N2 + fa];
+  }
+}
+
+template <int N, int DIR, typename T, bool VAR>
+__device__ __forceinline__ T cell_velocity(const AdvArgs<N, T> &a, long long cell, int l, int i)
+{
+  if constexpr (!VAR) return a.vel[DIR];
+  else {
+    constexpr int N3 = N * N * N;
+    const int dof = line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
+    return a.cell_vel[(cell * 3 + DIR) * N3 + dof];
+  }
+}
+
+// load the value face layers of the 6 neighbours (GLL nodal: the face layer
+// is the trace) into F[cc][f][fa]
+template <int N, typename T>
+__device__ __forceinline__ void load_face_layers(const AdvArgs<N, T> &a, const T *src, T *sF, long long cell0,
+                                                 int CB, int nf_first, int nf_count)
+{
+  constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
+  for (int it = threadIdx.x; it < CB * nf_count * N2; it += blockDim.x) {
+    const int cc = it / (nf_count * N2), fi = (it / N2) % nf_count, fa = it % N2;
+    const long long cl = cell0 + cc;
+    if (cl >= a.ncells) continue;
+    const int f = nf_first + fi, d = f / 2, s = f % 2;
+    const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, s) * N3;
+    const int lf = s ? 0 : p;
+    int of;
+    if (d == 0) of = face_dof<N, 0>(lf, fa);
+    else if (d == 1) of = face_dof<N, 1>(lf, fa);
+    else of = face_dof<N, 2>(lf, fa);
+    sF[(cc * 6 + fi) * N2 + fa] = src[nb + of];
+  }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void face_sweep(const Eo<N, T> &M, T *F, int narr, int l, bool lo)
+{
+  constexpr int N2 = N * N;
+  for (int it = l; it < narr * N; it += N2) {
+    T u[N], v[N];
+    T *fl = F + (it / N) * N2 + (lo ? (it % N) * N : (it % N));
+    const int st = lo ? 1 : N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = fl[i * st];
+    eo_apply<N, 1>(M, u, v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) fl[i * st] = v[i];
+  }
+}
+
+// forward (kind 0) and transposed weak form (kind 1): direction phase
+template <int N, int DIR, typename T, bool VAR, bool ADJ>
+__device__ __forceinline__ void adv_dir(const AdvArgs<N, T> &a, const T *Acell, const T *F, int l, long long cell,
+                                        T (&r)[N])
+{
+  const Tab<N, T> &tb = a.tab;
+  T q[N];
+  lds_read_line<N, N, DIR>(Acell, l, q);
+  const T c = a.ij[DIR] * a.jxw;
+  if constexpr (!ADJ) {
+    T g[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      g[i] = cell_velocity<N, DIR, T, VAR>(a, cell, l, i) * c * (line_weight<N, DIR>(tb.w, l, i) * q[i]);
+    eo_apply<N, -1>(tb.Db, g, r);
+  } else {
+    T g[N];
+    eo_apply<N, -1>(tb.Df, q, g);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      r[i] = cell_velocity<N, DIR, T, VAR>(a, cell, l, i) * c * (line_weight<N, DIR>(tb.w, l, i) * g[i]);
+  }
+  constexpr int N2 = N * N;
+  const T wf = tb.w[l % N] * tb.w[l / N] * a.jxwf[DIR];
+  T v0 = 0, v1 = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    v0 += tb.fv0[i] * q[i];
+    v1 += tb.fv1[i] * q[i];
+  }
+  T fl[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const T um = s ? v1 : v0;
+    const T up = F[(2 * DIR + s) * N2 + l];
+    const T an = normal_velocity<N, T, VAR>(a, cell, DIR, s, l);
+    if constexpr (!ADJ) {
+      // -0.5((u^- + u^+) a.n + |a.n| (u^- - u^+)), evaluation_dg_advect.h:403-404
+      fl[s] = T(-0.5) * ((um + up) * an + fabs(an) * (um - up)) * wf;
+    } else {
+      fl[s] = -(an > T(0) ? an : T(0)) * (um - up) * wf;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) r[i] += tb.fv0[i] * fl[0] + tb.fv1[i] * fl[1];
+}
+
+template <int N, typename T, int KIND, bool VAR, bool STAGE>
+__global__ void __launch_bounds__(adv_threads<N>()) advect_kernel(const AdvArgs<N, T> a)
+{
+  constexpr int CB = adv_cells_per_block<N>();
+  constexpr int N2 = N * N, N3 = N * N * N;
+  constexpr bool ADJ = KIND == 1;
+  __shared__ T sA[CB * N3];
+  __shared__ T sB[CB * N3];
+  __shared__ T sF[KIND == 2 ? 1 : CB * 6 * N2];
+  const Tab<N, T> &tb = a.tab;
+  const int t = threadIdx.x, c = t / N2, l = t % N2;
+  const long long cell0 = (long long)blockIdx.x * CB, cell = cell0 + c;
+  const bool act = c < CB && cell < a.ncells;
+  T *A = sA + (act ? c : 0) * N3;
+  T *B = sB + (act ? c : 0) * N3;
+  T *F = sF + (KIND != 2 && act ? c * 6 * N2 : 0);
+
+  if constexpr (KIND == 2) {
+    // M^{-1} x = Sinv^T W^{-1} Sinv x
+    if (act) {
+      T u[N], v[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
+      eo_apply<N, 1>(tb.Ib, u, v);
+      lds_write_line<N, N, 2>(A, l, v);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 0>(A, l, u);
+      eo_apply<N, 1>(tb.Ib, u, v);
+      lds_write_line<N, N, 0>(A, l, v);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(A, l, u);
+      eo_apply<N, 1>(tb.Ib, u, v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] /= a.jxw * line_weight<N, 1>(tb.w, l, i);
+      eo_apply<N, 1>(tb.If, v, u);
+      lds_write_line<N, N, 1>(A, l, u);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 0>(A, l, u);
+      eo_apply<N, 1>(tb.If, u, v);
+      lds_write_line<N, N, 0>(A, l, v);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 2>(A, l, u);
+      eo_apply<N, 1>(tb.If, u, v);
+#pragma unroll
+      for (int k = 0; k < N; ++k) a.dst[cell * N3 + k * N2 + l] = v[k];
+    }
+    return;
+  } else if constexpr (KIND == 3) {
+    // velocity gradient: A <- u at q-points, B <- mu at q-points,
+    // F[0..2] <- u^+ on the upper faces, F[3..5] <- mu^+ on the upper faces
+    if (act) {
+      T u[N], v[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 2>(A, l, v);
+#pragma unroll
+      for (int k = 0; k < N; ++k) u[k] = a.aux[cell * N3 + k * N2 + l];
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 2>(B, l, v);
+    }
+    for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
+      const int cc = it / (6 * N2), fi = (it / N2) % 6, fa = it % N2;
+      const long long cl = cell0 + cc;
+      if (cl >= a.ncells) continue;
+      const int d = fi % 3;
+      const T *vec = fi < 3 ? a.src : a.aux;
+      const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, 1) * N3;
+      int of;
+      if (d == 0) of = face_dof<N, 0>(0, fa);
+      else if (d == 1) of = face_dof<N, 1>(0, fa);
+      else of = face_dof<N, 2>(0, fa);
+      sF[(cc * 6 + fi) * N2 + fa] = vec[nb + of];
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 0>(A, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 0>(A, l, v);
+      lds_read_line<N, N, 0>(B, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 0>(B, l, v);
+      face_sweep<N, T>(tb.Sf, F, 6, l, true);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(A, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 1>(A, l, v);
+      lds_read_line<N, N, 1>(B, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 1>(B, l, v);
+      face_sweep<N, T>(tb.Sf, F, 6, l, false);
+    }
+    __syncthreads();
+    if (act) {
+      auto do_dir = [&](auto dirc) {
+        constexpr int DIR = decltype(dirc)::value;
+        T uq[N], mq[N], g[N];
+        lds_read_line<N, N, DIR>(A, l, uq);
+        lds_read_line<N, N, DIR>(B, l, mq);
+        eo_apply<N, -1>(tb.Df, mq, g);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const int dof = line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
+          a.dst[(cell * 3 + DIR) * N3 + dof] =
+              g[i] * a.ij[DIR] * uq[i] * (a.jxw * line_weight<N, DIR>(tb.w, l, i));
+        }
+        T um = 0, mm = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          um += tb.fv1[i] * uq[i];
+          mm += tb.fv1[i] * mq[i];
+        }
+        const T up = F[DIR * N2 + l], mp = F[(3 + DIR) * N2 + l];
+        const T b = a.face_vel[((long long)DIR * a.ncells + cell) * N2 + l];
+        const T sg = b > T(0) ? T(1) : (b < T(0) ? T(-1) : T(0));
+        const T uh = T(0.5) * (um + up) + T(0.5) * sg * (um - up);
+        a.dst2[((long long)DIR * a.ncells + cell) * N2 + l] =
+            -(mm - mp) * uh * (tb.w[l % N] * tb.w[l / N] * a.jxwf[DIR]);
+      };
+      do_dir(std::integral_constant<int, 0>());
+      do_dir(std::integral_constant<int, 1>());
+      do_dir(std::integral_constant<int, 2>());
+    }
+    return;
+  } else {
+    // ------------------------------------------------ L: z values + faces
+    if (act) {
+      T u[N], v[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 2>(A, l, v);
+    }
+    load_face_layers<N, T>(a, a.src, sF, cell0, CB, 0, 6);
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 0>(A, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 0>(A, l, v);
+      face_sweep<N, T>(tb.Sf, F, 6, l, true);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(A, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 1>(A, l, v);
+      face_sweep<N, T>(tb.Sf, F, 6, l, false);
+    }
+    __syncthreads();
+    if (act) {
+      T r[N];
+      adv_dir<N, 0, T, VAR, ADJ>(a, A, F, l, cell, r);
+      lds_write_line<N, N, 0>(B, l, r);
+    }
+    __syncthreads();
+    if (act) {
+      T r[N], b[N];
+      adv_dir<N, 1, T, VAR, ADJ>(a, A, F, l, cell, r);
+      lds_read_line<N, N, 1>(B, l, b);
+#pragma unroll
+      for (int i = 0; i < N; ++i) r[i] += b[i];
+      lds_write_line<N, N, 1>(B, l, r);
+    }
+    __syncthreads();
+    if (act) {
+      T r[N], b[N], o[N];
+      adv_dir<N, 2, T, VAR, ADJ>(a, A, F, l, cell, r);
+      lds_read_line<N, N, 2>(B, l, b);
+#pragma unroll
+      for (int i = 0; i < N; ++i) r[i] += b[i];
+      if constexpr (!ADJ) {
+        // inverse mass matrix by collocation, evaluation_dg_advect.h:619-624
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] /= a.jxw * line_weight<N, 2>(tb.w, l, i);
+        eo_apply<N, 1>(tb.If, r, o);
+      } else {
+        eo_apply<N, 1>(tb.Sb, r, o);
+      }
+      lds_write_line<N, N, 2>(A, l, o);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(A, l, u);
+      if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, v);
+      else eo_apply<N, 1>(tb.Sb, u, v);
+      lds_write_line<N, N, 1>(A, l, v);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], o[N];
+      lds_read_line<N, N, 0>(A, l, u);
+      if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, o);
+      else eo_apply<N, 1>(tb.Sb, u, o);
+      const long long g0 = cell * N3 + l * N;
+      if constexpr (STAGE) {
+        // evaluation_dg_advect.h:651-667
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const T vnp = a.upd[g0 + i] + a.c_np * o[i];
+          a.dst[g0 + i] = vnp;
+          if (a.c_tm != T(0)) a.tm[g0 + i] = vnp + a.c_tm * o[i];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.dst[g0 + i] = o[i];
+      }
+    }
+  }
+}
+
+template <int N, typename T>
+static Tab<N, T> adv_tab(const Tables1D &t)
+{
+  Tab<N, T> tb{};
+  tb.Sf = make_eo<N, T>(t.S, true);
+  tb.Sb = make_eo<N, T>(t.S, false);
+  tb.Df = make_eo<N, T>(t.D, true);
+  tb.Db = make_eo<N, T>(t.D, false);
+  tb.If = make_eo<N, T>(t.Sinv, true);
+  tb.Ib = make_eo<N, T>(t.Sinv, false);
+  for (int i = 0; i < N; ++i) {
+    tb.fv0[i] = T(t.fv0[i]);
+    tb.fv1[i] = T(t.fv1[i]);
+    tb.fd0[i] = T(t.fd0[i]);
+    tb.fd1[i] = T(t.fd1[i]);
+    tb.w[i] = T(t.weights[i]);
+  }
+  tb.hd = T(t.hermite_d);
+  return tb;
+}
+
+template <int N, typename T>
+static hipError_t launch_advect_n(const AdvectLaunch &L, hipStream_t s)
+{
+  AdvArgs<N, T> a{};
+  a.tab = adv_tab<N, T>(*L.tables);
+  a.src = (const T *)L.src;
+  a.upd = (const T *)L.upd;
+  a.aux = (const T *)L.aux;
+  a.dst = (T *)L.dst;
+  a.tm = (T *)L.tm;
+  a.dst2 = (T *)L.dst2;
+  a.c_np = T(L.c_np);
+  a.c_tm = T(L.c_tm);
+  a.cell_vel = (const T *)L.cell_vel;
+  a.face_vel = (const T *)L.face_vel;
+  const double jxw = 1. / (L.inv_jac[0] * L.inv_jac[1] * L.inv_jac[2]);
+  for (int d = 0; d < 3; ++d) {
+    a.vel[d] = T(L.vel[d]);
+    a.ij[d] = T(L.inv_jac[d]);
+    a.jxwf[d] = T(jxw * L.inv_jac[d]);
+  }
+  a.jxw = T(jxw);
+  a.ncells = L.ncells;
+  a.nx = L.nc[0];
+  a.ny = L.nc[1];
+  a.nz = L.nc[2];
+  constexpr int CB = adv_cells_per_block<N>();
+  const dim3 grid((unsigned)((L.ncells + CB - 1) / CB)), block(adv_threads<N>());
+  const bool var = L.cell_vel != nullptr;
+  switch (L.kind) {
+  case 0:
+    if (var) {
+      if (L.stage) hipLaunchKernelGGL((advect_kernel<N, T, 0, true, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((advect_kernel<N, T, 0, true, false>), grid, block, 0, s, a);
+    } else {
+      if (L.stage) hipLaunchKernelGGL((advect_kernel<N, T, 0, false, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((advect_kernel<N, T, 0, false, false>), grid, block, 0, s, a);
+    }
+    break;
+  case 1:
+    if (var) hipLaunchKernelGGL((advect_kernel<N, T, 1, true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((advect_kernel<N, T, 1, false, false>), grid, block, 0, s, a);
+    break;
+  case 2: hipLaunchKernelGGL((advect_kernel<N, T, 2, false, false>), grid, block, 0, s, a); break;
+  case 3: hipLaunchKernelGGL((advect_kernel<N, T, 3, true, false>), grid, block, 0, s, a); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t dispatch_advect(const AdvectLaunch &L, hipStream_t s)
+{
+  switch (L.tables->n) {
+#define SFK_CASE(NN) \
+  case NN: return launch_advect_n<NN, T>(L, s);
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_advect(const AdvectLaunch &L, hipStream_t s)
+{
+  return L.fp32 ? dispatch_advect<float>(L, s) : dispatch_advect<double>(L, s);
+}
+
+} // namespace sfk

@@ -1,0 +1,527 @@
+// Cell-batched sum-factorization kernels for the Laplacian family:
+//   Mode::Local  element-local Laplacian (EvaluationCellLaplacian,
+//                legacy/include/evaluation_cell_laplacian.h:264-503),
+//   Mode::Cont   continuous Q_p Laplacian on a structured node grid with
+//                homogeneous Dirichlet rows (gather / atomic scatter-add),
+//   Mode::DG     DG-SIPG on a Cartesian periodic mesh with the fused
+//                Chebyshev update (EvaluationDGLaplacian::do_inner_loop,
+//                legacy/include/evaluation_dg_laplacian.h:129-510).
+//
+// One workgroup processes CB cells; each cell gets N*N threads.  Thread l of
+// a cell owns line l of every direction (x-line (j,k), y-line (i,k), z-line
+// (i,j), all with index l = hi*N + lo), so each 1D sweep is a register-
+// resident even-odd contraction fed by one LDS read and one LDS write per
+// point.  Phase order (barriers between phases):
+//   L  global -> z-line registers, z value sweep -> LDS A;  DG: neighbour
+//      face layers (value + Hermite normal derivative) -> LDS F
+//   V1 x value sweep on A;  DG: face arrays, first in-plane sweep
+//   V2 y value sweep on A;  DG: face arrays, second in-plane sweep
+//   Gx x gradient, cell term, x-face terms, transposed gradient -> B
+//   Gy same for y, B += ...
+//   Gz same for z, + B, transposed z value sweep -> A
+//   By transposed y value sweep
+//   Bx transposed x value sweep -> global (DG: fused Chebyshev update;
+//      Cont: atomic scatter-add)
+#include "sfk_device.h"
+#include "sfk_kernels.h"
+
+namespace sfk {
+
+enum class Mode { Local = 0, Cont = 1, DG = 2 };
+
+template <int N>
+constexpr int cells_per_block()
+{
+  return 256 / (N * N) > 0 ? 256 / (N * N) : 1;
+}
+template <int N>
+constexpr int threads_per_block()
+{
+  return ((cells_per_block<N>() * N * N + 63) / 64) * 64;
+}
+
+template <int N, typename T>
+struct CellArgs {
+  Tab<N, T> tab;
+  const T *src;
+  T *dst;
+  const T *rhs, *diag; // Chebyshev
+  T *tm;
+  T c_np, c_tm;
+  const T *coef;   // per cell q-point coefficient (optional)
+  long long ncells;
+  int nx, ny, nz;  // cells per direction
+  int gx, gy, gz;  // node grid (continuous)
+  T cdiag[3];      // ij_d^2 * JxW
+  T ij[3];         // 1/h_d
+  T jxwf[3];       // JxW of a face normal to d (without weights)
+  T tau[3];        // penalty (p+1)^2 / h_d
+  int nrhs;
+  long long vstride; // elements between right-hand sides
+};
+
+template <int N, typename T, Mode MODE>
+__device__ __forceinline__ long long node_index(const CellArgs<N, T> &a, long long cell, int i, int j,
+                                                int k, bool &boundary)
+{
+  constexpr int p = N - 1;
+  const int cx = (int)(cell % a.nx);
+  const long long r = cell / a.nx;
+  const int cy = (int)(r % a.ny), cz = (int)(r / a.ny);
+  const int X = cx * p + i, Y = cy * p + j, Z = cz * p + k;
+  boundary = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
+  return ((long long)Z * a.gy + Y) * a.gx + X;
+}
+
+// face terms of SIPG along one line (Cartesian): the thread owns line l of
+// direction DIR, q[] are the values at the Gauss points of the line; adds
+// value and normal-derivative test contributions to r[] (the transposed
+// gradient sweep output).  Signs as in evaluation_dg_laplacian.h:237-252.
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void sipg_faces(const CellArgs<N, T> &a, const T (&q)[N], const T *F, int l,
+                                           T (&r)[N])
+{
+  constexpr int N2 = N * N;
+  const Tab<N, T> &tb = a.tab;
+  T v0 = 0, v1 = 0, d0 = 0, d1 = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    v0 += tb.fv0[i] * q[i];
+    v1 += tb.fv1[i] * q[i];
+    d0 += tb.fd0[i] * q[i];
+    d1 += tb.fd1[i] * q[i];
+  }
+  const T up0 = F[(2 * DIR) * N2 + l], dp0 = F[(6 + 2 * DIR) * N2 + l];
+  const T up1 = F[(2 * DIR + 1) * N2 + l], dp1 = F[(6 + 2 * DIR + 1) * N2 + l];
+  const T wf = tb.w[l % N] * tb.w[l / N];
+  const T ij = a.ij[DIR], jf = a.jxwf[DIR] * wf, tau = a.tau[DIR];
+  const T jump0 = v0 - up0, jump1 = v1 - up1;
+  const T tv0 = (T(0.5) * ij * (d0 + dp0) + jump0 * tau) * jf;
+  const T td0 = T(0.5) * ij * jf * jump0;
+  const T tv1 = (T(-0.5) * ij * (d1 + dp1) + jump1 * tau) * jf;
+  const T td1 = T(-0.5) * ij * jf * jump1;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    r[i] += tb.fv0[i] * tv0 + tb.fd0[i] * td0 + tb.fv1[i] * tv1 + tb.fd1[i] * td1;
+}
+
+// gradient phase of one direction: read Q line, collocation derivative,
+// cell term, faces, transposed derivative
+template <int N, int DIR, typename T, Mode MODE, bool COEF>
+__device__ __forceinline__ void grad_phase(const CellArgs<N, T> &a, const T *Acell, const T *F, int l,
+                                           long long cell, T (&r)[N])
+{
+  const Tab<N, T> &tb = a.tab;
+  T q[N], g[N];
+  lds_read_line<N, N, DIR>(Acell, l, q);
+  eo_apply<N, -1>(tb.Df, q, g);
+  const T c = a.cdiag[DIR];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    T f = c * line_weight<N, DIR>(tb.w, l, i);
+    if constexpr (COEF) {
+      const int dof = line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
+      f *= a.coef[cell * (N * N * N) + dof];
+    }
+    g[i] *= f;
+  }
+  eo_apply<N, -1>(tb.Db, g, r);
+  if constexpr (MODE == Mode::DG) sipg_faces<N, DIR, T>(a, q, F, l, r);
+}
+
+template <int N, typename T, Mode MODE, bool CHEBY, bool REFX, bool COEF>
+__global__ void __launch_bounds__(threads_per_block<N>()) cell_kernel(const CellArgs<N, T> a)
+{
+  constexpr int CB = cells_per_block<N>();
+  constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
+  constexpr bool FACES = MODE == Mode::DG;
+  __shared__ T sA[CB * N3];
+  __shared__ T sB[CB * N3];
+  __shared__ T sF[FACES ? CB * 12 * N2 : 1];
+
+  const Tab<N, T> &tb = a.tab;
+  const int t = threadIdx.x;
+  const int c = t / N2, l = t % N2;
+  const long long cell0 = (long long)blockIdx.x * CB;
+  const long long cell = cell0 + c;
+  const bool act = c < CB && cell < a.ncells;
+  T *A = sA + (act ? c : 0) * N3;
+  T *B = sB + (act ? c : 0) * N3;
+  T *F = sF + (FACES && act ? c * 12 * N2 : 0);
+
+  for (int rr = 0; rr < a.nrhs; ++rr) {
+    const T *src = a.src + rr * a.vstride;
+    T *dst = a.dst + rr * a.vstride;
+    // ---------------------------------------------------------------- L
+    if (act) {
+      T u[N], v[N];
+      if constexpr (MODE == Mode::Cont) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          bool bd;
+          const long long g = node_index<N, T, MODE>(a, cell, l % N, l / N, k, bd);
+          u[k] = bd ? T(0) : src[g];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) u[k] = src[cell * N3 + k * N2 + l];
+      }
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 2>(A, l, v);
+    }
+    if constexpr (FACES) {
+      // neighbour layers: value layer and Hermite normal derivative
+      // w0*(inner - face) (evaluation_dg_laplacian.h:196-203, 272-283)
+      for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
+        const int cc = it / (6 * N2), f = (it / N2) % 6, fa = it % N2;
+        const long long cl = cell0 + cc;
+        if (cl >= a.ncells) continue;
+        const int d = f / 2, s = f % 2;
+        const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, s) * N3;
+        const int lf = s ? 0 : p, li = s ? 1 : p - 1;
+        int of, oi;
+        if (d == 0) {
+          of = face_dof<N, 0>(lf, fa);
+          oi = face_dof<N, 0>(li, fa);
+        } else if (d == 1) {
+          of = face_dof<N, 1>(lf, fa);
+          oi = face_dof<N, 1>(li, fa);
+        } else {
+          of = face_dof<N, 2>(lf, fa);
+          oi = face_dof<N, 2>(li, fa);
+        }
+        const T uf = src[nb + of], ui = src[nb + oi];
+        const T w0 = s ? -tb.hd : tb.hd;
+        sF[(cc * 12 + f) * N2 + fa] = uf;
+        sF[(cc * 12 + 6 + f) * N2 + fa] = w0 * (ui - uf);
+      }
+    }
+    __syncthreads();
+    // --------------------------------------------------------------- V1
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 0>(A, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 0>(A, l, v);
+      if constexpr (FACES) {
+        for (int it = l; it < 12 * N; it += N2) {
+          T *fl = F + (it / N) * N2 + (it % N) * N; // lo-direction line
+#pragma unroll
+          for (int i = 0; i < N; ++i) u[i] = fl[i];
+          eo_apply<N, 1>(tb.Sf, u, v);
+#pragma unroll
+          for (int i = 0; i < N; ++i) fl[i] = v[i];
+        }
+      }
+    }
+    __syncthreads();
+    // --------------------------------------------------------------- V2
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(A, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 1>(A, l, v);
+      if constexpr (FACES) {
+        for (int it = l; it < 12 * N; it += N2) {
+          T *fl = F + (it / N) * N2 + (it % N); // hi-direction line
+#pragma unroll
+          for (int i = 0; i < N; ++i) u[i] = fl[i * N];
+          eo_apply<N, 1>(tb.Sf, u, v);
+#pragma unroll
+          for (int i = 0; i < N; ++i) fl[i * N] = v[i];
+        }
+      }
+    }
+    __syncthreads();
+    // --------------------------------------------------------------- Gx
+    if (act) {
+      T r[N];
+      grad_phase<N, 0, T, MODE, COEF>(a, A, F, l, cell, r);
+      lds_write_line<N, N, 0>(B, l, r);
+    }
+    __syncthreads();
+    // --------------------------------------------------------------- Gy
+    if (act) {
+      T r[N], b[N];
+      grad_phase<N, 1, T, MODE, COEF>(a, A, F, l, cell, r);
+      lds_read_line<N, N, 1>(B, l, b);
+#pragma unroll
+      for (int i = 0; i < N; ++i) r[i] += b[i];
+      lds_write_line<N, N, 1>(B, l, r);
+    }
+    __syncthreads();
+    // --------------------------------------------------------------- Gz
+    if (act) {
+      T r[N], b[N], o[N];
+      grad_phase<N, 2, T, MODE, COEF>(a, A, F, l, cell, r);
+      lds_read_line<N, N, 2>(B, l, b);
+#pragma unroll
+      for (int i = 0; i < N; ++i) r[i] += b[i];
+      eo_apply<N, 1>(tb.Sb, r, o);
+      lds_write_line<N, N, 2>(A, l, o);
+    }
+    __syncthreads();
+    // --------------------------------------------------------------- By
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(A, l, u);
+      eo_apply<N, 1>(tb.Sb, u, v);
+      lds_write_line<N, N, 1>(A, l, v);
+    }
+    __syncthreads();
+    // --------------------------------------------------------------- Bx
+    if (act) {
+      T u[N], o[N];
+      lds_read_line<N, N, 0>(A, l, u);
+      if constexpr (REFX)
+        eo_apply<N, 1>(tb.Sf, u, o); // evaluation_dg_laplacian.h:481-483
+      else
+        eo_apply<N, 1>(tb.Sb, u, o);
+      if constexpr (MODE == Mode::Cont) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          bool bd;
+          const long long g = node_index<N, T, MODE>(a, cell, i, l % N, l / N, bd);
+          if (!bd) atomicAdd(dst + g, o[i]);
+        }
+      } else if constexpr (CHEBY) {
+        // r = A u - rhs; tm = c_tm*tm + c_np*diag*r; dst = u - tm
+        // (evaluation_dg_laplacian.h:485-502)
+        const long long g0 = cell * N3 + l * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const long long g = g0 + i;
+          const T res = o[i] - a.rhs[g];
+          const T tmv = a.c_tm * a.tm[g] + a.c_np * a.diag[g] * res;
+          a.tm[g] = tmv;
+          dst[g] = src[g] - tmv;
+        }
+      } else {
+        const long long g0 = cell * N3 + l * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) dst[g0 + i] = o[i];
+      }
+    }
+    if (rr + 1 < a.nrhs) __syncthreads();
+  }
+}
+
+// Diagonal of the Cartesian Laplacian (local or continuous, optionally with
+// a per-q coefficient): diag_i = sum_d sum_q c_d k_q w_q (dphi_i/dxi_d)^2
+// prod_{e!=d} phi_i^2 -- a sum-factorized contraction of the weights with
+// the squared 1D matrices (G1 = S D: basis derivative at Gauss points).
+template <int N, typename T>
+struct DiagArgs {
+  Eo<N, T> S2b, G2b; // backward contractions with S^2 and G1^2
+  T w[N];
+  const T *coef;
+  T *diag;
+  long long ncells;
+  int nx, ny, nz, gx, gy, gz;
+  T cdiag[3];
+  int cont;
+};
+
+template <int N, typename T>
+__global__ void __launch_bounds__(threads_per_block<N>()) diag_kernel(const DiagArgs<N, T> a)
+{
+  constexpr int CB = cells_per_block<N>();
+  constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
+  __shared__ T sA[CB * N3];
+  __shared__ T sR[CB * N3];
+  const int t = threadIdx.x, c = t / N2, l = t % N2;
+  const long long cell = (long long)blockIdx.x * CB + c;
+  const bool act = c < CB && cell < a.ncells;
+  T *A = sA + (act ? c : 0) * N3;
+  T *R = sR + (act ? c : 0) * N3;
+  for (int d = 0; d < 3; ++d) {
+    if (act) {
+      // weights c_d w_q k_q on x-line l
+      T u[N], v[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        T f = a.cdiag[d] * line_weight<N, 0>(a.w, l, i);
+        if (a.coef) f *= a.coef[cell * N3 + l * N + i];
+        u[i] = f;
+      }
+      if (d == 0) eo_apply<N, 1>(a.G2b, u, v);
+      else eo_apply<N, 1>(a.S2b, u, v);
+      lds_write_line<N, N, 0>(A, l, v);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(A, l, u);
+      if (d == 1) eo_apply<N, 1>(a.G2b, u, v);
+      else eo_apply<N, 1>(a.S2b, u, v);
+      lds_write_line<N, N, 1>(A, l, v);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N], r[N];
+      lds_read_line<N, N, 2>(A, l, u);
+      if (d == 2) eo_apply<N, 1>(a.G2b, u, v);
+      else eo_apply<N, 1>(a.S2b, u, v);
+      if (d > 0) lds_read_line<N, N, 2>(R, l, r);
+#pragma unroll
+      for (int i = 0; i < N; ++i) r[i] = d > 0 ? r[i] + v[i] : v[i];
+      lds_write_line<N, N, 2>(R, l, r);
+    }
+    __syncthreads();
+  }
+  if (act) {
+    T r[N];
+    lds_read_line<N, N, 2>(R, l, r); // z-line l = j*N + i
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      if (a.cont) {
+        const int cx = (int)(cell % a.nx);
+        const long long rr = cell / a.nx;
+        const int cy = (int)(rr % a.ny), cz = (int)(rr / a.ny);
+        const int X = cx * p + l % N, Y = cy * p + l / N, Z = cz * p + k;
+        const bool bd = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
+        if (!bd) atomicAdd(a.diag + ((long long)Z * a.gy + Y) * a.gx + X, r[k]);
+      } else {
+        a.diag[cell * N3 + k * N2 + l] = r[k];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- launch
+template <int N, typename T>
+static Tab<N, T> make_tab(const Tables1D &t)
+{
+  Tab<N, T> tb{};
+  tb.Sf = make_eo<N, T>(t.S, true);
+  tb.Sb = make_eo<N, T>(t.S, false);
+  tb.Df = make_eo<N, T>(t.D, true);
+  tb.Db = make_eo<N, T>(t.D, false);
+  tb.If = make_eo<N, T>(t.Sinv, true);
+  tb.Ib = make_eo<N, T>(t.Sinv, false);
+  for (int i = 0; i < N; ++i) {
+    tb.fv0[i] = T(t.fv0[i]);
+    tb.fv1[i] = T(t.fv1[i]);
+    tb.fd0[i] = T(t.fd0[i]);
+    tb.fd1[i] = T(t.fd1[i]);
+    tb.w[i] = T(t.weights[i]);
+  }
+  tb.hd = T(t.hermite_d);
+  return tb;
+}
+
+template <int N, typename T>
+static hipError_t launch_cell_n(const CellLaunch &L, hipStream_t stream)
+{
+  CellArgs<N, T> a{};
+  a.tab = make_tab<N, T>(*L.tables);
+  a.src = (const T *)L.src;
+  a.dst = (T *)L.dst;
+  a.rhs = (const T *)L.rhs;
+  a.diag = (const T *)L.diag;
+  a.tm = (T *)L.tm;
+  a.c_np = T(L.c_np);
+  a.c_tm = T(L.c_tm);
+  a.coef = (const T *)L.coef;
+  a.ncells = L.ncells;
+  a.nx = L.nc[0];
+  a.ny = L.nc[1];
+  a.nz = L.nc[2];
+  a.gx = L.nc[0] * (N - 1) + 1;
+  a.gy = L.nc[1] * (N - 1) + 1;
+  a.gz = L.nc[2] * (N - 1) + 1;
+  const double jxw = 1. / (L.inv_jac[0] * L.inv_jac[1] * L.inv_jac[2]);
+  for (int d = 0; d < 3; ++d) {
+    a.cdiag[d] = T(L.inv_jac[d] * L.inv_jac[d] * jxw);
+    a.ij[d] = T(L.inv_jac[d]);
+    a.jxwf[d] = T(jxw * L.inv_jac[d]);
+    a.tau[d] = T(double(N * N) * L.inv_jac[d]);
+  }
+  a.nrhs = L.nrhs;
+  a.vstride = L.vstride;
+  constexpr int CB = cells_per_block<N>();
+  const unsigned blocks = (unsigned)((L.ncells + CB - 1) / CB);
+  const dim3 grid(blocks), block(threads_per_block<N>());
+  if (L.mode == 0) {
+    if (L.coef)
+      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Local, false, false, true>), grid, block, 0, stream, a);
+    else
+      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Local, false, false, false>), grid, block, 0, stream, a);
+  } else if (L.mode == 1) {
+    if (L.coef)
+      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Cont, false, false, true>), grid, block, 0, stream, a);
+    else
+      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Cont, false, false, false>), grid, block, 0, stream, a);
+  } else {
+    if (L.cheby) {
+      if (L.refx)
+        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, true, true, false>), grid, block, 0, stream, a);
+      else
+        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, true, false, false>), grid, block, 0, stream, a);
+    } else {
+      if (L.refx)
+        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, false, true, false>), grid, block, 0, stream, a);
+      else
+        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, false, false, false>), grid, block, 0, stream, a);
+    }
+  }
+  return hipGetLastError();
+}
+
+template <int N, typename T>
+static hipError_t launch_diag_n(const CellLaunch &L, hipStream_t stream)
+{
+  DiagArgs<N, T> a{};
+  const Tables1D &t = *L.tables;
+  std::vector<double> S2(N * N), G2(N * N);
+  for (int i = 0; i < N; ++i)
+    for (int q = 0; q < N; ++q) {
+      double g = 0.;
+      for (int m = 0; m < N; ++m) g += t.S[i * N + m] * t.D[m * N + q];
+      S2[i * N + q] = t.S[i * N + q] * t.S[i * N + q];
+      G2[i * N + q] = g * g;
+    }
+  a.S2b = make_eo<N, T>(S2, false);
+  a.G2b = make_eo<N, T>(G2, false);
+  for (int i = 0; i < N; ++i) a.w[i] = T(t.weights[i]);
+  a.coef = (const T *)L.coef;
+  a.diag = (T *)L.dst;
+  a.ncells = L.ncells;
+  a.nx = L.nc[0];
+  a.ny = L.nc[1];
+  a.nz = L.nc[2];
+  a.gx = L.nc[0] * (N - 1) + 1;
+  a.gy = L.nc[1] * (N - 1) + 1;
+  a.gz = L.nc[2] * (N - 1) + 1;
+  const double jxw = 1. / (L.inv_jac[0] * L.inv_jac[1] * L.inv_jac[2]);
+  for (int d = 0; d < 3; ++d) a.cdiag[d] = T(L.inv_jac[d] * L.inv_jac[d] * jxw);
+  a.cont = L.mode == 1;
+  constexpr int CB = cells_per_block<N>();
+  const unsigned blocks = (unsigned)((L.ncells + CB - 1) / CB);
+  hipLaunchKernelGGL((diag_kernel<N, T>), dim3(blocks), dim3(threads_per_block<N>()), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t dispatch_cell(const CellLaunch &L, hipStream_t s, bool diag)
+{
+  switch (L.tables->n) {
+#define SFK_CASE(NN) \
+  case NN: return diag ? launch_diag_n<NN, T>(L, s) : launch_cell_n<NN, T>(L, s);
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_cell(const CellLaunch &L, hipStream_t stream)
+{
+  return L.fp32 ? dispatch_cell<float>(L, stream, false) : dispatch_cell<double>(L, stream, false);
+}
+
+hipError_t launch_cell_diagonal(const CellLaunch &L, hipStream_t stream)
+{
+  return L.fp32 ? dispatch_cell<float>(L, stream, true) : dispatch_cell<double>(L, stream, true);
+}
+
+} // namespace sfk

@@ -1,0 +1,157 @@
+// Device-side building blocks for the gfx950 sum-factorization kernels:
+// even-odd 1D contraction tables (passed by value as kernel arguments, so
+// every lane reads the same coefficient from scalar registers), the even-odd
+// contraction of one register-resident line, and small index helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace sfk {
+
+// Even-odd form of the contraction out[o] = sum_k K[k][o] in[k] of an N x N
+// matrix K with the centro-symmetry K[N-1-k][N-1-o] = SIG * K[k][o]
+// (SIG = +1 for basis values, -1 for derivatives).  With e_k = in[k] +
+// in[N-1-k], o_k = in[k] - in[N-1-k] (k < N/2):
+//   P_o = sum_k A[k][o] e_k + C[o] in[mid],  Q_o = sum_k B[k][o] o_k,
+//   out[o] = P_o + Q_o,  out[N-1-o] = SIG (P_o - Q_o),
+// and for odd N the middle output sum_k E[k] (SIG>0 ? e_k : o_k) + Kmm in[mid].
+// Halves the multiplications of the dense contraction (the reference kernel
+// legacy/include/matrix_vector_kernel.h:30-185 uses the same symmetry; this
+// form keeps the middle terms of every matrix, so it is exact also for
+// bases that are not nodal at x = 1/2).
+template <int N, typename T>
+struct Eo {
+  static constexpr int M = N / 2 > 0 ? N / 2 : 1;
+  T A[M][M];
+  T B[M][M];
+  T C[M];
+  T E[M];
+  T Kmm;
+};
+
+// Build from a dense matrix M[i*n+q]; forward: K = M (out[q] = sum_i
+// M[i][q] in[i]), backward: K = M^T (out[i] = sum_q M[i][q] in[q]).
+template <int N, typename T>
+inline Eo<N, T> make_eo(const std::vector<double> &Mat, bool forward)
+{
+  Eo<N, T> e{};
+  auto K = [&](int k, int o) { return forward ? Mat[k * N + o] : Mat[o * N + k]; };
+  const int mid = N / 2;
+  for (int k = 0; k < mid; ++k)
+    for (int o = 0; o < mid; ++o) {
+      e.A[k][o] = T(0.5 * (K(k, o) + K(N - 1 - k, o)));
+      e.B[k][o] = T(0.5 * (K(k, o) - K(N - 1 - k, o)));
+    }
+  if (N % 2 == 1) {
+    for (int o = 0; o < mid; ++o) e.C[o] = T(K(mid, o));
+    for (int k = 0; k < mid; ++k) e.E[k] = T(K(k, mid));
+    e.Kmm = T(K(mid, mid));
+  }
+  return e;
+}
+
+template <int N, int SIG, typename T>
+__device__ __forceinline__ void eo_apply(const Eo<N, T> &t, const T (&in)[N], T (&out)[N])
+{
+  constexpr int mid = N / 2;
+  T e[Eo<N, T>::M], o[Eo<N, T>::M];
+#pragma unroll
+  for (int k = 0; k < mid; ++k) {
+    e[k] = in[k] + in[N - 1 - k];
+    o[k] = in[k] - in[N - 1 - k];
+  }
+  T r[N];
+#pragma unroll
+  for (int q = 0; q < mid; ++q) {
+    T p = t.A[0][q] * e[0];
+    T s = t.B[0][q] * o[0];
+#pragma unroll
+    for (int k = 1; k < mid; ++k) {
+      p = t.A[k][q] * e[k] + p;
+      s = t.B[k][q] * o[k] + s;
+    }
+    if constexpr (N % 2 == 1) p = t.C[q] * in[mid] + p;
+    r[q] = p + s;
+    r[N - 1 - q] = SIG > 0 ? p - s : s - p;
+  }
+  if constexpr (N % 2 == 1) {
+    T m = SIG > 0 ? t.Kmm * in[mid] : T(0);
+#pragma unroll
+    for (int k = 0; k < mid; ++k) m = t.E[k] * (SIG > 0 ? e[k] : o[k]) + m;
+    r[mid] = m;
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) out[q] = r[q];
+}
+
+// 1D tables of one operator at compile-time size N and precision T.
+template <int N, typename T>
+struct Tab {
+  Eo<N, T> Sf, Sb;  // basis values: forward (dof -> quad), backward (quad -> dof)
+  Eo<N, T> Df, Db;  // collocation derivative at Gauss points, and its transpose
+  Eo<N, T> If, Ib;  // Gauss Lagrange at GLL points (inverse of Sf), and transpose
+  T fv0[N], fv1[N], fd0[N], fd1[N]; // l_q(0), l_q(1), l_q'(0), l_q'(1)
+  T w[N];           // Gauss weights
+  T hd;             // phi_0'(0) of the Hermite-like basis
+};
+
+// Line helpers inside one cell's LDS block (x fastest, row pitch NP).
+template <int N, int NP, int DIR>
+__device__ __forceinline__ int line_base(int l)
+{
+  if constexpr (DIR == 0) return l * NP;                      // l = k*N + j
+  else if constexpr (DIR == 1) return (l / N) * N * NP + (l % N); // l = k*N + i
+  else return (l / N) * NP + (l % N);                         // l = j*N + i
+}
+template <int N, int NP, int DIR>
+constexpr int line_stride()
+{
+  return DIR == 0 ? 1 : (DIR == 1 ? NP : N * NP);
+}
+
+template <int N, int NP, int DIR, typename T>
+__device__ __forceinline__ void lds_read_line(const T *cell, int l, T (&v)[N])
+{
+  const int b = line_base<N, NP, DIR>(l);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = cell[b + i * line_stride<N, NP, DIR>()];
+}
+template <int N, int NP, int DIR, typename T>
+__device__ __forceinline__ void lds_write_line(T *cell, int l, const T (&v)[N])
+{
+  const int b = line_base<N, NP, DIR>(l);
+#pragma unroll
+  for (int i = 0; i < N; ++i) cell[b + i * line_stride<N, NP, DIR>()] = v[i];
+}
+
+// quadrature weight of point i on line l of direction DIR
+template <int N, int DIR, typename T>
+__device__ __forceinline__ T line_weight(const T *w, int l, int i)
+{
+  return w[i] * w[l % N] * w[l / N];
+}
+
+// dof index (inside a cell) of point `layer` along DIR at face point fa
+template <int N, int DIR>
+__device__ __forceinline__ int face_dof(int layer, int fa)
+{
+  if constexpr (DIR == 0) return fa * N + layer;
+  else if constexpr (DIR == 1) return (fa / N) * N * N + layer * N + (fa % N);
+  else return layer * N * N + fa;
+}
+
+__device__ __forceinline__ long long neighbor_cell(long long c, int nx, int ny, int nz, int dir, int side)
+{
+  int ix = (int)(c % nx);
+  long long r = c / nx;
+  int iy = (int)(r % ny);
+  int iz = (int)(r / ny);
+  if (dir == 0) ix = side ? (ix + 1 == nx ? 0 : ix + 1) : (ix == 0 ? nx - 1 : ix - 1);
+  else if (dir == 1) iy = side ? (iy + 1 == ny ? 0 : iy + 1) : (iy == 0 ? ny - 1 : iy - 1);
+  else iz = side ? (iz + 1 == nz ? 0 : iz + 1) : (iz == 0 ? nz - 1 : iz - 1);
+  return ((long long)iz * ny + iy) * nx + ix;
+}
+
+} // namespace sfk

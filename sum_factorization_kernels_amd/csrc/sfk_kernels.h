@@ -1,0 +1,101 @@
+// Launch descriptors of the HIP kernels (host side).  Each launcher picks
+// the compile-time degree instantiation and the precision at run time.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sfk_internal.h"
+
+namespace sfk {
+
+struct CellLaunch {
+  const Tables1D *tables = nullptr;
+  bool fp32 = false;
+  int mode = 0; // 0 element-local, 1 continuous (Dirichlet), 2 DG-SIPG Cartesian
+  bool cheby = false, refx = false;
+  const void *src = nullptr;
+  void *dst = nullptr;
+  const void *rhs = nullptr, *diag = nullptr;
+  void *tm = nullptr;
+  double c_np = 0., c_tm = 0.;
+  const void *coef = nullptr;
+  long long ncells = 0;
+  int nc[3] = {1, 1, 1};
+  double inv_jac[3] = {1., 1., 1.};
+  int nrhs = 1;
+  long long vstride = 0;
+};
+
+hipError_t launch_cell(const CellLaunch &L, hipStream_t stream);
+hipError_t launch_cell_diagonal(const CellLaunch &L, hipStream_t stream);
+
+// DG-SIPG with per-quadrature-point metric (deformed mesh)
+struct SipgLaunch {
+  const Tables1D *tables = nullptr;
+  bool fp32 = false;
+  bool cheby = false;
+  bool diagonal = false; // write the operator diagonal into dst
+  const void *src = nullptr;
+  void *dst = nullptr;
+  const void *rhs = nullptr, *diag = nullptr;
+  void *tm = nullptr;
+  double c_np = 0., c_tm = 0.;
+  const void *cell_metric = nullptr, *face_metric = nullptr;
+  long long ncells = 0;
+  int nc[3] = {1, 1, 1};
+};
+hipError_t launch_sipg_general(const SipgLaunch &L, hipStream_t stream);
+
+// DG advection (forward, RK stage, adjoint, mass inverse, velocity gradient)
+struct AdvectLaunch {
+  const Tables1D *tables = nullptr;
+  bool fp32 = false;
+  int kind = 0; // 0 forward M^{-1}L u, 1 L^T y (adjoint after M^{-1}), 2 M^{-1} x, 3 velocity gradient
+  bool stage = false;
+  const void *src = nullptr, *upd = nullptr, *aux = nullptr;
+  void *dst = nullptr, *tm = nullptr, *dst2 = nullptr;
+  double c_np = 0., c_tm = 0.;
+  const void *cell_vel = nullptr, *face_vel = nullptr;
+  double vel[3] = {0., 0., 0.};
+  long long ncells = 0;
+  int nc[3] = {1, 1, 1};
+  double inv_jac[3] = {1., 1., 1.};
+};
+hipError_t launch_advect(const AdvectLaunch &L, hipStream_t stream);
+
+// continuous Laplacian: coefficient gradient g_q = JxW grad u . grad lambda
+struct CoefGradLaunch {
+  const Tables1D *tables = nullptr;
+  bool fp32 = false;
+  const void *u = nullptr, *lambda = nullptr;
+  void *grad = nullptr;
+  long long ncells = 0;
+  int nc[3] = {1, 1, 1};
+  double inv_jac[3] = {1., 1., 1.};
+  int nrhs = 1;
+  long long vstride = 0;
+};
+hipError_t launch_coef_gradient(const CoefGradLaunch &L, hipStream_t stream);
+
+// sine-deformed mesh metric (cell and face), computed on the device
+hipError_t launch_sine_metric(const Tables1D &t, bool fp32, const int nc[3], double amp,
+                              void *cell_metric, void *face_metric, hipStream_t stream);
+
+// vector kernels
+hipError_t vec_fill(bool fp32, void *x, long long n, double v, hipStream_t s);
+hipError_t vec_copy_convert(bool to_fp32, const void *x, void *y, long long n, hipStream_t s);
+// partial dot products: writes ceil(n / chunk) FP64 partials into `partials`
+hipError_t vec_dot(bool fp32, const void *x, const void *y, long long n, double *result_dev,
+                   double *partials, int n_partials, hipStream_t s);
+// y = a*x + b*y (FP64 scalars from device memory if ptrs given)
+hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, void *y, hipStream_t s);
+hipError_t vec_cheby_update(bool fp32, long long n, const void *au, const void *rhs, const void *sol_old,
+                            const void *diag, void *sol_new, void *tm, double c_np, double c_tm,
+                            hipStream_t s);
+hipError_t vec_zero_dirichlet(bool fp32, void *x, const int g[3], hipStream_t s);
+hipError_t layout_convert(int degree, const int nc[3], int lanes, const double *in, double *out,
+                          bool to_canonical, hipStream_t s);
+hipError_t apply_1d(int nn, const std::vector<double> &M, int sym, bool forward, long long n_lines,
+                    long long line_stride, int stride, const double *in, double *out, hipStream_t s);
+
+} // namespace sfk

@@ -1,0 +1,53 @@
+// The operator handle behind the opaque sfk_op of include/sfk.h.
+#pragma once
+
+#include "sfk.h"
+#include "sfk_internal.h"
+#include "sfk_kernels.h"
+
+struct sfk_op {
+  sfk_op_desc d{};
+  sfk::Tables1D tab;
+  long long ncells = 0;
+  long long nvec = 0; // elements of one vector
+  long long ncq = 0;  // cell quadrature points (all cells)
+  long long nfq = 0;  // face quadrature points per direction (all cells)
+  int grid[3] = {0, 0, 0};
+  // device data owned by the handle (handle precision)
+  void *cell_metric = nullptr, *face_metric = nullptr, *coef = nullptr;
+  void *cell_vel = nullptr, *face_vel = nullptr;
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace sfk {
+
+inline void fill_sipg_launch(const sfk_op *op, SipgLaunch &L)
+{
+  L.tables = &op->tab;
+  L.fp32 = op->d.precision == SFK_FP32;
+  L.ncells = op->ncells;
+  for (int d = 0; d < 3; ++d) L.nc[d] = op->d.n_cells[d];
+  L.cell_metric = op->cell_metric;
+  L.face_metric = op->face_metric;
+}
+
+inline void fill_advect_launch(const sfk_op *op, AdvectLaunch &L)
+{
+  L.tables = &op->tab;
+  L.fp32 = op->d.precision == SFK_FP32;
+  L.ncells = op->ncells;
+  for (int d = 0; d < 3; ++d) {
+    L.nc[d] = op->d.n_cells[d];
+    L.inv_jac[d] = op->d.inv_jac[d];
+    L.vel[d] = op->d.velocity[d];
+  }
+  if (op->d.flags & SFK_FLAG_VARIABLE_VELOCITY) {
+    L.cell_vel = op->cell_vel;
+    L.face_vel = op->face_vel;
+  }
+}
+
+size_t elem_size(const sfk_op *op);
+
+} // namespace sfk

@@ -1,0 +1,219 @@
+// Vector kernels: fill, convert, FP64-accumulated dot products, axpby,
+// the Chebyshev vector update for operators whose matvec cannot fuse it
+// (continuous Q_p: the assembled value of a node is complete only after all
+// cells have scattered), Dirichlet masking, layout converters between the
+// canonical and the reference's lane-interleaved layout, and the batched 1D
+// even-odd kernel.  All grid-stride loops; every block exits.
+#include "sfk_device.h"
+#include "sfk_kernels.h"
+
+namespace sfk {
+
+namespace {
+constexpr int kThreads = 256;
+
+inline unsigned grid_for(long long n, int per_thread = 1)
+{
+  long long b = (n + (long long)kThreads * per_thread - 1) / ((long long)kThreads * per_thread);
+  if (b < 1) b = 1;
+  if (b > 65535LL * 8) b = 65535LL * 8;
+  return (unsigned)b;
+}
+
+template <typename T>
+__global__ void fill_kernel(T *x, long long n, T v)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+template <typename S, typename D>
+__global__ void convert_kernel(const S *x, D *y, long long n)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = D(x[i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) dot_partial_kernel(const T *x, const T *y, long long n, double *partials)
+{
+  __shared__ double red[kThreads];
+  double s = 0.;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    s += double(x[i]) * double(y[i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kThreads) dot_final_kernel(const double *partials, int n, double *result)
+{
+  __shared__ double red[kThreads];
+  double s = 0.;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *result = red[0];
+}
+
+template <typename T>
+__global__ void axpby_kernel(long long n, T a, const T *x, T b, T *y)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
+template <typename T>
+__global__ void cheby_kernel(long long n, const T *au, const T *rhs, const T *sol_old, const T *diag,
+                             T *sol_new, T *tm, T c_np, T c_tm)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const T res = au[i] - rhs[i];
+    const T t = c_tm * tm[i] + c_np * diag[i] * res;
+    tm[i] = t;
+    sol_new[i] = sol_old[i] - t;
+  }
+}
+
+template <typename T>
+__global__ void dirichlet_kernel(T *x, int gx, int gy, int gz)
+{
+  const long long n = (long long)gx * gy * gz;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int X = (int)(i % gx);
+    const long long r = i / gx;
+    const int Y = (int)(r % gy), Z = (int)(r / gy);
+    if (X == 0 || Y == 0 || Z == 0 || X == gx - 1 || Y == gy - 1 || Z == gz - 1) x[i] = T(0);
+  }
+}
+
+__global__ void layout_kernel(int n3, int nx, int ny, int nz, int lanes, const double *in, double *out,
+                              int to_canonical)
+{
+  const long long total = (long long)nx * ny * nz * n3;
+  const int nxb = nx / lanes;
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total; g += (long long)gridDim.x * blockDim.x) {
+    const int dof = (int)(g % n3);
+    const long long cell = g / n3;
+    const int ix = (int)(cell % nx);
+    const long long r = cell / nx;
+    const int iy = (int)(r % ny), iz = (int)(r / ny);
+    const long long il =
+        (((long long)iz * ny + iy) * nxb + ix / lanes) * (long long)lanes * n3 + (long long)dof * lanes + ix % lanes;
+    if (to_canonical) out[g] = in[il];
+    else out[il] = in[g];
+  }
+}
+
+template <int N>
+__global__ void apply1d_kernel(Eo<N, double> t, int sym, long long n_lines, long long line_stride, int stride,
+                               const double *in, double *out)
+{
+  for (long long li = blockIdx.x * (long long)blockDim.x + threadIdx.x; li < n_lines; li += (long long)gridDim.x * blockDim.x) {
+    double u[N], v[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = in[li * line_stride + (long long)i * stride];
+    if (sym > 0) eo_apply<N, 1>(t, u, v);
+    else eo_apply<N, -1>(t, u, v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[li * line_stride + (long long)i * stride] = v[i];
+  }
+}
+} // namespace
+
+hipError_t vec_fill(bool fp32, void *x, long long n, double v, hipStream_t s)
+{
+  if (fp32) hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, (float *)x, n, (float)v);
+  else hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, (double *)x, n, v);
+  return hipGetLastError();
+}
+
+hipError_t vec_copy_convert(bool to_fp32, const void *x, void *y, long long n, hipStream_t s)
+{
+  if (to_fp32)
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n, 4)), dim3(kThreads), 0, s, (const double *)x, (float *)y, n);
+  else
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n, 4)), dim3(kThreads), 0, s, (const float *)x, (double *)y, n);
+  return hipGetLastError();
+}
+
+hipError_t vec_dot(bool fp32, const void *x, const void *y, long long n, double *result_dev, double *partials,
+                   int n_partials, hipStream_t s)
+{
+  unsigned g = grid_for(n, 8);
+  if ((int)g > n_partials) g = n_partials;
+  if (fp32)
+    hipLaunchKernelGGL(dot_partial_kernel<float>, dim3(g), dim3(kThreads), 0, s, (const float *)x, (const float *)y, n, partials);
+  else
+    hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(g), dim3(kThreads), 0, s, (const double *)x, (const double *)y, n, partials);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
+  return hipGetLastError();
+}
+
+hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, void *y, hipStream_t s)
+{
+  if (fp32)
+    hipLaunchKernelGGL(axpby_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, (float)a, (const float *)x, (float)b, (float *)y);
+  else
+    hipLaunchKernelGGL(axpby_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, a, (const double *)x, b, (double *)y);
+  return hipGetLastError();
+}
+
+hipError_t vec_cheby_update(bool fp32, long long n, const void *au, const void *rhs, const void *sol_old,
+                            const void *diag, void *sol_new, void *tm, double c_np, double c_tm, hipStream_t s)
+{
+  if (fp32)
+    hipLaunchKernelGGL(cheby_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, (const float *)au,
+                       (const float *)rhs, (const float *)sol_old, (const float *)diag, (float *)sol_new, (float *)tm,
+                       (float)c_np, (float)c_tm);
+  else
+    hipLaunchKernelGGL(cheby_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, (const double *)au,
+                       (const double *)rhs, (const double *)sol_old, (const double *)diag, (double *)sol_new,
+                       (double *)tm, c_np, c_tm);
+  return hipGetLastError();
+}
+
+hipError_t vec_zero_dirichlet(bool fp32, void *x, const int g[3], hipStream_t s)
+{
+  const long long n = (long long)g[0] * g[1] * g[2];
+  if (fp32) hipLaunchKernelGGL(dirichlet_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, (float *)x, g[0], g[1], g[2]);
+  else hipLaunchKernelGGL(dirichlet_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, (double *)x, g[0], g[1], g[2]);
+  return hipGetLastError();
+}
+
+hipError_t layout_convert(int degree, const int nc[3], int lanes, const double *in, double *out, bool to_canonical,
+                          hipStream_t s)
+{
+  const int n = degree + 1;
+  const long long total = (long long)nc[0] * nc[1] * nc[2] * n * n * n;
+  hipLaunchKernelGGL(layout_kernel, dim3(grid_for(total, 4)), dim3(kThreads), 0, s, n * n * n, nc[0], nc[1], nc[2],
+                     lanes, in, out, to_canonical ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t apply_1d(int nn, const std::vector<double> &M, int sym, bool forward, long long n_lines,
+                    long long line_stride, int stride, const double *in, double *out, hipStream_t s)
+{
+  const unsigned g = grid_for(n_lines);
+  switch (nn) {
+#define SFK_CASE(NN)                                                                                          \
+  case NN:                                                                                                    \
+    hipLaunchKernelGGL(apply1d_kernel<NN>, dim3(g), dim3(kThreads), 0, s, make_eo<NN, double>(M, forward), sym, \
+                       n_lines, line_stride, stride, in, out);                                                \
+    break;
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+} // namespace sfk

@@ -77,6 +77,7 @@ def ref():
                                          ctypes.c_double, ctypes.c_double, _dp, _dp]
         lib.ref_dg_advect.argtypes = [ctypes.c_int, _up, _dp, ctypes.c_int, _dp, _dp, _dp,
                                       ctypes.c_double, ctypes.c_double, _dp, _dp]
+        lib.ref_advect_kat.argtypes = [ctypes.c_int, _up, ctypes.c_int, _dp, _dp, _dp, _dp]
         _ref = lib
     return _ref
 
@@ -211,3 +212,62 @@ def ref_dg_advect(degree, nc, inv_jac, src, upd=None, tm=None, c_np=0.0, c_tm=0.
                              _ptr(out_np), _ptr(out_tm))
     assert rc == 0
     return out_np, out_tm
+
+
+# ------------------------------------------------------- extended oracle
+def cont_laplacian(degree, nc, inv_jac, src, coef=None):
+    dst = np.zeros_like(src)
+    oracle().sfko_cont_laplacian(degree, _i3(nc), _d3(inv_jac), _ptr(coef), _ptr(src), _ptr(dst))
+    return dst
+
+
+def cont_laplacian_coef_grad(degree, nc, inv_jac, u_all, lam_all, n_rhs=1):
+    n3 = (degree + 1) ** 3
+    g = np.zeros(int(np.prod(nc)) * n3)
+    oracle().sfko_cont_laplacian_coef_grad(degree, _i3(nc), _d3(inv_jac), n_rhs, _ptr(u_all), _ptr(lam_all),
+                                           _ptr(g))
+    return g
+
+
+def cartesian_metric(degree, nc, inv_jac):
+    n = degree + 1
+    ncell = int(np.prod(nc))
+    cm = np.zeros(ncell * 6 * n ** 3)
+    fm = np.zeros(3 * ncell * 8 * n * n)
+    oracle().sfko_cartesian_metric(degree, _i3(nc), _d3(inv_jac), _ptr(cm), _ptr(fm))
+    return cm, fm
+
+
+def dg_sipg_general(degree, nc, cm, fm, src):
+    dst = np.zeros_like(src)
+    oracle().sfko_dg_sipg_general(degree, _i3(nc), _ptr(cm), _ptr(fm), _ptr(src), _ptr(dst))
+    return dst
+
+
+def dg_advect_var(degree, nc, inv_jac, cv, fv, src):
+    dst = np.zeros_like(src)
+    oracle().sfko_dg_advect_var(degree, _i3(nc), _d3(inv_jac), _ptr(cv), _ptr(fv), _ptr(src), _ptr(dst))
+    return dst
+
+
+def dg_advect_var_transpose(degree, nc, inv_jac, cv, fv, src):
+    dst = np.zeros_like(src)
+    oracle().sfko_dg_advect_var_transpose(degree, _i3(nc), _d3(inv_jac), _ptr(cv), _ptr(fv), _ptr(src),
+                                          _ptr(dst))
+    return dst
+
+
+def dg_mass_inverse(degree, nc, inv_jac, src):
+    dst = np.zeros_like(src)
+    oracle().sfko_dg_mass_inverse(degree, _i3(nc), _d3(inv_jac), _ptr(src), _ptr(dst))
+    return dst
+
+
+def dg_advect_velocity_grad(degree, nc, inv_jac, fv, u, lam):
+    n = degree + 1
+    ncell = int(np.prod(nc))
+    gc = np.zeros(ncell * 3 * n ** 3)
+    gf = np.zeros(3 * ncell * n * n)
+    oracle().sfko_dg_advect_velocity_grad(degree, _i3(nc), _d3(inv_jac), _ptr(fv), _ptr(u), _ptr(lam), _ptr(gc),
+                                          _ptr(gf))
+    return gc, gf
