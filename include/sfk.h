@@ -146,6 +146,9 @@ sfk_status sfk_grad_velocity(sfk_op *op, const void *u, const void *lambda, void
 sfk_status sfk_grad_coefficient(sfk_op *op, const void *u, const void *lambda, void *grad_k,
                                 void *stream);
 
+/* b_i = integral of f phi_i (continuous Laplacian; Dirichlet rows 0) */
+sfk_status sfk_op_rhs_constant(sfk_op *op, double f, void *b, void *stream);
+
 /* ---- solver (continuous Laplacian) ------------------------------------
  * Preconditioned CG in FP64 with a Chebyshev(degree) point-Jacobi
  * preconditioner whose bounds come from a Lanczos estimate of

@@ -77,6 +77,7 @@ def lib():
         L.sfk_cheby_step.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp]
         L.sfk_rk_stage.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp]
         L.sfk_compute_diagonal.argtypes = [_vp, _vp, _vp]
+        L.sfk_op_rhs_constant.argtypes = [_vp, ctypes.c_double, _vp, _vp]
         L.sfk_grad_velocity.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.sfk_grad_coefficient.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.sfk_solve_cg_cheby.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -231,6 +232,12 @@ class Operator:
         d = self.new_vector()[: self.vector_size]
         check(lib().sfk_compute_diagonal(self.handle, _dptr(d), _stream(stream)))
         return d
+
+    def rhs_constant(self, f=1.0, stream=None):
+        """b_i = integral of f phi_i (continuous Laplacian, Dirichlet rows 0)"""
+        b = self.new_vector()[: self.vector_size]
+        check(lib().sfk_op_rhs_constant(self.handle, float(f), _dptr(b), _stream(stream)))
+        return b
 
     def grad_velocity(self, u, lam, stream=None):
         import torch

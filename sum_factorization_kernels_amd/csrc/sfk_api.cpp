@@ -424,6 +424,22 @@ sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream)
   return SFK_ERR_UNSUPPORTED;
 }
 
+sfk_status sfk_op_rhs_constant(sfk_op *op, double f, void *b, void *stream)
+{
+  SFK_REQUIRE(op && b, SFK_ERR_INVALID_ARG, "sfk_op_rhs_constant: null argument");
+  SFK_REQUIRE(op->d.kind == SFK_CONT_LAPLACE, SFK_ERR_UNSUPPORTED, "sfk_op_rhs_constant: continuous Laplacian only");
+  hipStream_t s = (hipStream_t)stream;
+  CellLaunch L;
+  fill_cell_launch(op, L);
+  L.mode = 1;
+  L.dst = b;
+  L.load = true;
+  L.load_value = f;
+  SFK_CHECK_HIP(vec_fill(op->d.precision == SFK_FP32, b, op->nvec, 0., s));
+  SFK_CHECK_HIP(launch_cell_diagonal(L, s));
+  return SFK_OK;
+}
+
 sfk_status sfk_grad_velocity(sfk_op *op, const void *u, const void *lambda, void *grad_cell, void *grad_face,
                              void *stream)
 {

@@ -313,6 +313,9 @@ __global__ void __launch_bounds__(threads_per_block<N>()) cell_kernel(const Cell
 template <int N, typename T>
 struct DiagArgs {
   Eo<N, T> S2b, G2b; // backward contractions with S^2 and G1^2
+  Eo<N, T> S1b;      // load vector: backward contraction with S
+  int load;          // 1: b_i = sum_q f JxW_q phi_i(x_q) instead of the diagonal
+  T fval, jxw;
   T w[N];
   const T *coef;
   T *diag;
@@ -334,17 +337,18 @@ __global__ void __launch_bounds__(threads_per_block<N>()) diag_kernel(const Diag
   const bool act = c < CB && cell < a.ncells;
   T *A = sA + (act ? c : 0) * N3;
   T *R = sR + (act ? c : 0) * N3;
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < (a.load ? 1 : 3); ++d) {
     if (act) {
       // weights c_d w_q k_q on x-line l
       T u[N], v[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        T f = a.cdiag[d] * line_weight<N, 0>(a.w, l, i);
-        if (a.coef) f *= a.coef[cell * N3 + l * N + i];
+        T f = (a.load ? a.fval * a.jxw : a.cdiag[d]) * line_weight<N, 0>(a.w, l, i);
+        if (a.coef && !a.load) f *= a.coef[cell * N3 + l * N + i];
         u[i] = f;
       }
-      if (d == 0) eo_apply<N, 1>(a.G2b, u, v);
+      if (a.load) eo_apply<N, 1>(a.S1b, u, v);
+      else if (d == 0) eo_apply<N, 1>(a.G2b, u, v);
       else eo_apply<N, 1>(a.S2b, u, v);
       lds_write_line<N, N, 0>(A, l, v);
     }
@@ -352,7 +356,8 @@ __global__ void __launch_bounds__(threads_per_block<N>()) diag_kernel(const Diag
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      if (d == 1) eo_apply<N, 1>(a.G2b, u, v);
+      if (a.load) eo_apply<N, 1>(a.S1b, u, v);
+      else if (d == 1) eo_apply<N, 1>(a.G2b, u, v);
       else eo_apply<N, 1>(a.S2b, u, v);
       lds_write_line<N, N, 1>(A, l, v);
     }
@@ -360,7 +365,8 @@ __global__ void __launch_bounds__(threads_per_block<N>()) diag_kernel(const Diag
     if (act) {
       T u[N], v[N], r[N];
       lds_read_line<N, N, 2>(A, l, u);
-      if (d == 2) eo_apply<N, 1>(a.G2b, u, v);
+      if (a.load) eo_apply<N, 1>(a.S1b, u, v);
+      else if (d == 2) eo_apply<N, 1>(a.G2b, u, v);
       else eo_apply<N, 1>(a.S2b, u, v);
       if (d > 0) lds_read_line<N, N, 2>(R, l, r);
 #pragma unroll
@@ -386,6 +392,136 @@ __global__ void __launch_bounds__(threads_per_block<N>()) diag_kernel(const Diag
       }
     }
   }
+}
+
+// Gradient of lambda^T A(k) u with respect to the per-q coefficient of the
+// continuous Laplacian: g_q = sum_rhs JxW_q sum_d ij_d^2 (du/dxi_d)(dlambda/dxi_d)
+// at every cell quadrature point.
+template <int N, typename T>
+struct CoefGradArgs {
+  Tab<N, T> tab;
+  const T *u, *lam;
+  T *grad;
+  long long ncells;
+  int nx, ny, nz, gx, gy, gz;
+  T cdiag[3];
+  int nrhs;
+  long long vstride;
+};
+
+template <int N, typename T>
+__global__ void __launch_bounds__(threads_per_block<N>()) coef_grad_kernel(const CoefGradArgs<N, T> a)
+{
+  constexpr int CB = cells_per_block<N>();
+  constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
+  __shared__ T sU[CB * N3];
+  __shared__ T sL[CB * N3];
+  __shared__ T sG[CB * N3];
+  const Tab<N, T> &tb = a.tab;
+  const int t = threadIdx.x, c = t / N2, l = t % N2;
+  const long long cell = (long long)blockIdx.x * CB + c;
+  const bool act = c < CB && cell < a.ncells;
+  T *U = sU + (act ? c : 0) * N3;
+  T *Lm = sL + (act ? c : 0) * N3;
+  T *G = sG + (act ? c : 0) * N3;
+  const int cx = act ? (int)(cell % a.nx) : 0;
+  const long long rr0 = act ? cell / a.nx : 0;
+  const int cy = (int)(rr0 % a.ny), cz = (int)(rr0 / a.ny);
+  for (int r = 0; r < a.nrhs; ++r) {
+    if (act) {
+      T u[N], v[N], w[N], z[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const int X = cx * p + l % N, Y = cy * p + l / N, Z = cz * p + k;
+        const bool bd = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
+        const long long g = ((long long)Z * a.gy + Y) * a.gx + X + r * a.vstride;
+        u[k] = bd ? T(0) : a.u[g];
+        w[k] = bd ? T(0) : a.lam[g];
+      }
+      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, w, z);
+      lds_write_line<N, N, 2>(U, l, v);
+      lds_write_line<N, N, 2>(Lm, l, z);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 0>(U, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 0>(U, l, v);
+      lds_read_line<N, N, 0>(Lm, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 0>(Lm, l, v);
+    }
+    __syncthreads();
+    if (act) {
+      T u[N], v[N];
+      lds_read_line<N, N, 1>(U, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 1>(U, l, v);
+      lds_read_line<N, N, 1>(Lm, l, u);
+      eo_apply<N, 1>(tb.Sf, u, v);
+      lds_write_line<N, N, 1>(Lm, l, v);
+    }
+    __syncthreads();
+    auto dir_term = [&](auto dirc, bool first) {
+      constexpr int DIR = decltype(dirc)::value;
+      T u[N], w[N], gu[N], gw[N], acc[N];
+      lds_read_line<N, N, DIR>(U, l, u);
+      lds_read_line<N, N, DIR>(Lm, l, w);
+      eo_apply<N, -1>(tb.Df, u, gu);
+      eo_apply<N, -1>(tb.Df, w, gw);
+      if (first) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) acc[i] = T(0);
+      } else {
+        lds_read_line<N, N, DIR>(G, l, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] += a.cdiag[DIR] * line_weight<N, DIR>(tb.w, l, i) * gu[i] * gw[i];
+      lds_write_line<N, N, DIR>(G, l, acc);
+    };
+    if (act) dir_term(std::integral_constant<int, 0>(), r == 0);
+    __syncthreads();
+    if (act) dir_term(std::integral_constant<int, 1>(), false);
+    __syncthreads();
+    if (act) dir_term(std::integral_constant<int, 2>(), false);
+    __syncthreads();
+  }
+  if (act) {
+    T v[N];
+    lds_read_line<N, N, 0>(G, l, v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) a.grad[cell * N3 + l * N + i] = v[i];
+  }
+}
+
+template <int N, typename T>
+static Tab<N, T> make_tab(const Tables1D &t);
+
+template <int N, typename T>
+static hipError_t launch_coef_grad_n(const CoefGradLaunch &L, hipStream_t s)
+{
+  CoefGradArgs<N, T> a{};
+  a.tab = make_tab<N, T>(*L.tables);
+  a.u = (const T *)L.u;
+  a.lam = (const T *)L.lambda;
+  a.grad = (T *)L.grad;
+  a.ncells = L.ncells;
+  a.nx = L.nc[0];
+  a.ny = L.nc[1];
+  a.nz = L.nc[2];
+  a.gx = L.nc[0] * (N - 1) + 1;
+  a.gy = L.nc[1] * (N - 1) + 1;
+  a.gz = L.nc[2] * (N - 1) + 1;
+  const double jxw = 1. / (L.inv_jac[0] * L.inv_jac[1] * L.inv_jac[2]);
+  for (int d = 0; d < 3; ++d) a.cdiag[d] = T(L.inv_jac[d] * L.inv_jac[d] * jxw);
+  a.nrhs = L.nrhs;
+  a.vstride = L.vstride;
+  constexpr int CB = cells_per_block<N>();
+  hipLaunchKernelGGL((coef_grad_kernel<N, T>), dim3((unsigned)((L.ncells + CB - 1) / CB)),
+                     dim3(threads_per_block<N>()), 0, s, a);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- launch
@@ -483,6 +619,9 @@ static hipError_t launch_diag_n(const CellLaunch &L, hipStream_t stream)
     }
   a.S2b = make_eo<N, T>(S2, false);
   a.G2b = make_eo<N, T>(G2, false);
+  a.S1b = make_eo<N, T>(t.S, false);
+  a.load = L.load ? 1 : 0;
+  a.fval = T(L.load_value);
   for (int i = 0; i < N; ++i) a.w[i] = T(t.weights[i]);
   a.coef = (const T *)L.coef;
   a.diag = (T *)L.dst;
@@ -495,6 +634,7 @@ static hipError_t launch_diag_n(const CellLaunch &L, hipStream_t stream)
   a.gz = L.nc[2] * (N - 1) + 1;
   const double jxw = 1. / (L.inv_jac[0] * L.inv_jac[1] * L.inv_jac[2]);
   for (int d = 0; d < 3; ++d) a.cdiag[d] = T(L.inv_jac[d] * L.inv_jac[d] * jxw);
+  a.jxw = T(jxw);
   a.cont = L.mode == 1;
   constexpr int CB = cells_per_block<N>();
   const unsigned blocks = (unsigned)((L.ncells + CB - 1) / CB);
@@ -522,6 +662,23 @@ hipError_t launch_cell(const CellLaunch &L, hipStream_t stream)
 hipError_t launch_cell_diagonal(const CellLaunch &L, hipStream_t stream)
 {
   return L.fp32 ? dispatch_cell<float>(L, stream, true) : dispatch_cell<double>(L, stream, true);
+}
+
+template <typename T>
+static hipError_t dispatch_coef_grad(const CoefGradLaunch &L, hipStream_t s)
+{
+  switch (L.tables->n) {
+#define SFK_CASE(NN) \
+  case NN: return launch_coef_grad_n<NN, T>(L, s);
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_coef_gradient(const CoefGradLaunch &L, hipStream_t stream)
+{
+  return L.fp32 ? dispatch_coef_grad<float>(L, stream) : dispatch_coef_grad<double>(L, stream);
 }
 
 } // namespace sfk

@@ -24,6 +24,8 @@ struct CellLaunch {
   double inv_jac[3] = {1., 1., 1.};
   int nrhs = 1;
   long long vstride = 0;
+  bool load = false;       // launch_cell_diagonal: load vector of a constant instead
+  double load_value = 0.;
 };
 
 hipError_t launch_cell(const CellLaunch &L, hipStream_t stream);

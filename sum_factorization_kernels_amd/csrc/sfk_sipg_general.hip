@@ -1,0 +1,595 @@
+// DG-SIPG with a per-quadrature-point metric (deformed periodic mesh) and
+// the device-side generator of the metric of a sine-deformed mesh.
+//
+// Operator (the reference has the Cartesian special case only,
+// legacy/include/evaluation_dg_laplacian.h:129-510):
+//   a(u,v) = sum_K (G grad^u, grad^v)_K
+//          + sum_F [ -{du/dn}[v] - {dv/dn}[u] + sigma [u][v] ]_F
+// with G = JxW J^{-1}J^{-T} per cell q-point and, per face q-point,
+// aX = JxW_f J_X^{-1} n for both sides and sigma = tau JxW_f (layouts in
+// include/sfk.h).  The neighbour's value and normal derivative come from
+// its two nodal layers next to the face (Hermite-like basis,
+// evaluation_dg_laplacian.h:196-203), its tangential derivatives from the
+// in-plane collocation derivative of the interpolated face values.  All face
+// test functions, value and gradient, are injected at the cell's Gauss
+// points through l_q(s), so one transposed collocation-derivative sweep per
+// direction integrates cell and face terms together.
+//
+// One workgroup = CB cells, N*N threads per cell, thread l owns line l of
+// every direction.  Phases (barrier after each):
+//   L   u z-lines -> values; neighbour layers -> F (value, normal derivative)
+//   V1  x value sweep; F first in-plane sweep
+//   V2  y value sweep; F second in-plane sweep
+//   G   reference gradient Gx, Gy, Gz (one line of each per thread)
+//   E   traces of u and grad u on all 6 faces, neighbour data, face metric
+//       -> face test coefficients kept in registers (tv, td[3] per face)
+//   M   z-line owner: T = G grad^u (cell metric) + z-face gradient tests
+//   Iy  y-line owner: y-face gradient tests into T
+//   Ix  x-line owner: x-face gradient tests, x transposed sweep + x value
+//       tests -> R
+//   By  y transposed sweep + y value tests + R -> R
+//   Bz  z transposed sweep + z value tests + R, z value sweep^T -> Q
+//   Sy, Sx  transposed value sweeps; output (+ fused Chebyshev update)
+#include "sfk_device.h"
+#include "sfk_kernels.h"
+
+namespace sfk {
+
+template <int N>
+constexpr int gen_cells_per_block()
+{
+  // 4 N^3 + 12 N^2 values of LDS per cell
+  return N <= 3 ? 256 / (N * N) : (N == 4 ? 12 : (N == 5 ? 8 : (N == 6 ? 5 : (N == 7 ? 3 : (N == 8 ? 2 : 1)))));
+}
+template <int N>
+constexpr int gen_threads()
+{
+  return ((gen_cells_per_block<N>() * N * N + 63) / 64) * 64;
+}
+
+template <int N, typename T>
+struct GenArgs {
+  Tab<N, T> tab;
+  T Dd[N][N]; // dense collocation derivative D[m][q] (tangential derivatives)
+  const T *src;
+  T *dst;
+  const T *rhs, *diag;
+  T *tm;
+  T c_np, c_tm;
+  const T *cm; // [cell][6][q]
+  const T *fm; // [d][cell][8][fa]
+  long long ncells;
+  long long nitems; // cells (matvec) or cells * N^3 (diagonal)
+  int nx, ny, nz;
+};
+
+template <int N, int DIR>
+__device__ __forceinline__ int pt(int l, int i)
+{
+  return line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
+}
+
+// traces, neighbour data and face metric of one direction -> test coefficients
+template <int N, int DIR, typename T, bool DIAG>
+__device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, const T *Gx, const T *Gy, const T *Gz,
+                                           const T *F, int l, long long cell, T (&tv)[2], T (&td)[2][3])
+{
+  constexpr int N2 = N * N;
+  const Tab<N, T> &tb = a.tab;
+  T q[N], g[3][N];
+  lds_read_line<N, N, DIR>(Q, l, q);
+  lds_read_line<N, N, DIR>(Gx, l, g[0]);
+  lds_read_line<N, N, DIR>(Gy, l, g[1]);
+  lds_read_line<N, N, DIR>(Gz, l, g[2]);
+  constexpr int LO = DIR == 0 ? 1 : 0, HI = DIR == 2 ? 1 : 2;
+  const int llo = l % N, lhi = l / N;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const T *fv = s ? tb.fv1 : tb.fv0;
+    const T *fd = s ? tb.fd1 : tb.fd0;
+    T um = 0, gm[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      um += fv[i] * q[i];
+      gm[DIR] += fd[i] * q[i];
+      gm[LO] += fv[i] * g[LO][i];
+      gm[HI] += fv[i] * g[HI][i];
+    }
+    T up = 0, gp[3] = {0, 0, 0};
+    if constexpr (!DIAG) {
+      const T *FV = F + (2 * DIR + s) * N2;
+      up = FV[l];
+      gp[DIR] = F[(6 + 2 * DIR + s) * N2 + l];
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        gp[LO] += a.Dd[m][llo] * FV[lhi * N + m];
+        gp[HI] += a.Dd[m][lhi] * FV[m * N + llo];
+      }
+    }
+    const long long owner = s ? cell : neighbor_cell(cell, a.nx, a.ny, a.nz, DIR, 0);
+    const T *fmp = a.fm + ((long long)DIR * a.ncells + owner) * 8 * N2 + l;
+    T ao[3], an[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      ao[e] = s ? fmp[e * N2] : fmp[(3 + e) * N2];
+      an[e] = s ? fmp[(3 + e) * N2] : fmp[e * N2];
+    }
+    const T sigma = fmp[7 * N2];
+    const T sn = s ? T(1) : T(-1);
+    T dn = 0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) dn += ao[e] * gm[e] + an[e] * gp[e];
+    const T jump = um - up;
+    tv[s] = T(-0.5) * sn * dn + sigma * jump;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) td[s][e] = T(-0.5) * jump * sn * ao[e];
+  }
+}
+
+// inject the gradient tests of direction DIR's faces into T (own d-line)
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void inject_grad(const Tab<N, T> &tb, T *Tx, T *Ty, T *Tz, int l, const T (&td)[2][3])
+{
+  T *Ts[3] = {Tx, Ty, Tz};
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    T v[N];
+    lds_read_line<N, N, DIR>(Ts[e], l, v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += tb.fv0[i] * td[0][e] + tb.fv1[i] * td[1][e];
+    lds_write_line<N, N, DIR>(Ts[e], l, v);
+  }
+}
+
+template <int N, typename T, bool CHEBY, bool DIAG>
+__global__ void __launch_bounds__(gen_threads<N>()) sipg_general_kernel(const GenArgs<N, T> a)
+{
+  constexpr int CB = gen_cells_per_block<N>();
+  constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
+  __shared__ T sQ[CB * N3];
+  __shared__ T sG[3][CB * N3];
+  __shared__ T sF[DIAG ? 1 : CB * 12 * N2];
+  const Tab<N, T> &tb = a.tab;
+  const int t = threadIdx.x, c = t / N2, l = t % N2;
+  const long long item0 = (long long)blockIdx.x * CB, item = item0 + c;
+  const bool act = c < CB && item < a.nitems;
+  const long long cell = DIAG ? item / N3 : item;
+  const int unit = DIAG ? (int)(item % N3) : 0; // diagonal: dof of the unit vector
+  T *Q = sQ + (act ? c : 0) * N3;
+  T *Gx = sG[0] + (act ? c : 0) * N3;
+  T *Gy = sG[1] + (act ? c : 0) * N3;
+  T *Gz = sG[2] + (act ? c : 0) * N3;
+  T *F = sF + (!DIAG && act ? c * 12 * N2 : 0);
+
+  // ------------------------------------------------------------------ L
+  if (act) {
+    T u[N], v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      if constexpr (DIAG) u[k] = (k * N2 + l == unit) ? T(1) : T(0);
+      else u[k] = a.src[cell * N3 + k * N2 + l];
+    }
+    eo_apply<N, 1>(tb.Sf, u, v);
+    lds_write_line<N, N, 2>(Q, l, v);
+  }
+  if constexpr (!DIAG) {
+    for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
+      const int cc = it / (6 * N2), f = (it / N2) % 6, fa = it % N2;
+      const long long cl = item0 + cc;
+      if (cl >= a.ncells) continue;
+      const int d = f / 2, s = f % 2;
+      const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, s) * N3;
+      const int lf = s ? 0 : p, li = s ? 1 : p - 1;
+      int of, oi;
+      if (d == 0) {
+        of = face_dof<N, 0>(lf, fa);
+        oi = face_dof<N, 0>(li, fa);
+      } else if (d == 1) {
+        of = face_dof<N, 1>(lf, fa);
+        oi = face_dof<N, 1>(li, fa);
+      } else {
+        of = face_dof<N, 2>(lf, fa);
+        oi = face_dof<N, 2>(li, fa);
+      }
+      const T uf = a.src[nb + of], ui = a.src[nb + oi];
+      sF[(cc * 12 + f) * N2 + fa] = uf;
+      sF[(cc * 12 + 6 + f) * N2 + fa] = (s ? -tb.hd : tb.hd) * (ui - uf);
+    }
+  }
+  __syncthreads();
+  // ----------------------------------------------------------------- V1
+  if (act) {
+    T u[N], v[N];
+    lds_read_line<N, N, 0>(Q, l, u);
+    eo_apply<N, 1>(tb.Sf, u, v);
+    lds_write_line<N, N, 0>(Q, l, v);
+    if constexpr (!DIAG) {
+      for (int it = l; it < 12 * N; it += N2) {
+        T *fl = F + (it / N) * N2 + (it % N) * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) u[i] = fl[i];
+        eo_apply<N, 1>(tb.Sf, u, v);
+#pragma unroll
+        for (int i = 0; i < N; ++i) fl[i] = v[i];
+      }
+    }
+  }
+  __syncthreads();
+  // ----------------------------------------------------------------- V2
+  if (act) {
+    T u[N], v[N];
+    lds_read_line<N, N, 1>(Q, l, u);
+    eo_apply<N, 1>(tb.Sf, u, v);
+    lds_write_line<N, N, 1>(Q, l, v);
+    if constexpr (!DIAG) {
+      for (int it = l; it < 12 * N; it += N2) {
+        T *fl = F + (it / N) * N2 + (it % N);
+#pragma unroll
+        for (int i = 0; i < N; ++i) u[i] = fl[i * N];
+        eo_apply<N, 1>(tb.Sf, u, v);
+#pragma unroll
+        for (int i = 0; i < N; ++i) fl[i * N] = v[i];
+      }
+    }
+  }
+  __syncthreads();
+  // ------------------------------------------------------------------ G
+  if (act) {
+    T u[N], v[N];
+    lds_read_line<N, N, 0>(Q, l, u);
+    eo_apply<N, -1>(tb.Df, u, v);
+    lds_write_line<N, N, 0>(Gx, l, v);
+    lds_read_line<N, N, 1>(Q, l, u);
+    eo_apply<N, -1>(tb.Df, u, v);
+    lds_write_line<N, N, 1>(Gy, l, v);
+    lds_read_line<N, N, 2>(Q, l, u);
+    eo_apply<N, -1>(tb.Df, u, v);
+    lds_write_line<N, N, 2>(Gz, l, v);
+  }
+  __syncthreads();
+  // ------------------------------------------------------------------ E
+  T tvx[2], tdx[2][3], tvy[2], tdy[2][3], tvz[2], tdz[2][3];
+  if (act) {
+    face_terms<N, 0, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, tvx, tdx);
+    face_terms<N, 1, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, tvy, tdy);
+    face_terms<N, 2, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, tvz, tdz);
+  }
+  __syncthreads();
+  // ------------------------------------------------------------------ M
+  if (act) {
+    T gx[N], gy[N], gz[N];
+    lds_read_line<N, N, 2>(Gx, l, gx);
+    lds_read_line<N, N, 2>(Gy, l, gy);
+    lds_read_line<N, N, 2>(Gz, l, gz);
+    const T *m = a.cm + cell * 6 * N3;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const int q = k * N2 + l;
+      const T g00 = m[q], g11 = m[N3 + q], g22 = m[2 * N3 + q], g01 = m[3 * N3 + q], g02 = m[4 * N3 + q],
+              g12 = m[5 * N3 + q];
+      const T x = gx[k], y = gy[k], z = gz[k];
+      gx[k] = g00 * x + g01 * y + g02 * z + tb.fv0[k] * tdz[0][0] + tb.fv1[k] * tdz[1][0];
+      gy[k] = g01 * x + g11 * y + g12 * z + tb.fv0[k] * tdz[0][1] + tb.fv1[k] * tdz[1][1];
+      gz[k] = g02 * x + g12 * y + g22 * z + tb.fv0[k] * tdz[0][2] + tb.fv1[k] * tdz[1][2];
+    }
+    lds_write_line<N, N, 2>(Gx, l, gx);
+    lds_write_line<N, N, 2>(Gy, l, gy);
+    lds_write_line<N, N, 2>(Gz, l, gz);
+  }
+  __syncthreads();
+  // ----------------------------------------------------------------- Iy
+  if (act) inject_grad<N, 1, T>(tb, Gx, Gy, Gz, l, tdy);
+  __syncthreads();
+  // ----------------------------------------------------------------- Ix
+  if (act) {
+    T v[N], r[N];
+    lds_read_line<N, N, 0>(Gx, l, v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += tb.fv0[i] * tdx[0][0] + tb.fv1[i] * tdx[1][0];
+    eo_apply<N, -1>(tb.Db, v, r);
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] += tb.fv0[i] * tvx[0] + tb.fv1[i] * tvx[1];
+    lds_write_line<N, N, 0>(Q, l, r);
+    // tangential x-face tests into T_y, T_z
+#pragma unroll
+    for (int e = 1; e < 3; ++e) {
+      T *Te = e == 1 ? Gy : Gz;
+      lds_read_line<N, N, 0>(Te, l, v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] += tb.fv0[i] * tdx[0][e] + tb.fv1[i] * tdx[1][e];
+      lds_write_line<N, N, 0>(Te, l, v);
+    }
+  }
+  __syncthreads();
+  // ----------------------------------------------------------------- By
+  if (act) {
+    T v[N], r[N], b[N];
+    lds_read_line<N, N, 1>(Gy, l, v);
+    eo_apply<N, -1>(tb.Db, v, r);
+    lds_read_line<N, N, 1>(Q, l, b);
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] += b[i] + tb.fv0[i] * tvy[0] + tb.fv1[i] * tvy[1];
+    lds_write_line<N, N, 1>(Q, l, r);
+  }
+  __syncthreads();
+  // ----------------------------------------------------------------- Bz
+  if (act) {
+    T v[N], r[N], b[N];
+    lds_read_line<N, N, 2>(Gz, l, v);
+    eo_apply<N, -1>(tb.Db, v, r);
+    lds_read_line<N, N, 2>(Q, l, b);
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] += b[i] + tb.fv0[i] * tvz[0] + tb.fv1[i] * tvz[1];
+    eo_apply<N, 1>(tb.Sb, r, v);
+    lds_write_line<N, N, 2>(Q, l, v);
+  }
+  __syncthreads();
+  // ----------------------------------------------------------------- Sy
+  if (act) {
+    T u[N], v[N];
+    lds_read_line<N, N, 1>(Q, l, u);
+    eo_apply<N, 1>(tb.Sb, u, v);
+    lds_write_line<N, N, 1>(Q, l, v);
+  }
+  __syncthreads();
+  // ----------------------------------------------------------------- Sx
+  if (act) {
+    T u[N], o[N];
+    lds_read_line<N, N, 0>(Q, l, u);
+    eo_apply<N, 1>(tb.Sb, u, o);
+    if constexpr (DIAG) {
+      if (unit / N == l) a.dst[cell * N3 + unit] = o[unit % N];
+    } else {
+      const long long g0 = cell * N3 + l * N;
+      if constexpr (CHEBY) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const long long g = g0 + i;
+          const T res = o[i] - a.rhs[g];
+          const T tmv = a.c_tm * a.tm[g] + a.c_np * a.diag[g] * res;
+          a.tm[g] = tmv;
+          a.dst[g] = a.src[g] - tmv;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.dst[g0 + i] = o[i];
+      }
+    }
+  }
+}
+
+template <int N, typename T>
+static hipError_t launch_general_n(const SipgLaunch &L, hipStream_t s)
+{
+  const Tables1D &t = *L.tables;
+  GenArgs<N, T> a{};
+  a.tab.Sf = make_eo<N, T>(t.S, true);
+  a.tab.Sb = make_eo<N, T>(t.S, false);
+  a.tab.Df = make_eo<N, T>(t.D, true);
+  a.tab.Db = make_eo<N, T>(t.D, false);
+  for (int i = 0; i < N; ++i) {
+    a.tab.fv0[i] = T(t.fv0[i]);
+    a.tab.fv1[i] = T(t.fv1[i]);
+    a.tab.fd0[i] = T(t.fd0[i]);
+    a.tab.fd1[i] = T(t.fd1[i]);
+    a.tab.w[i] = T(t.weights[i]);
+    for (int q = 0; q < N; ++q) a.Dd[i][q] = T(t.D[i * N + q]);
+  }
+  a.tab.hd = T(t.hermite_d);
+  a.src = (const T *)L.src;
+  a.dst = (T *)L.dst;
+  a.rhs = (const T *)L.rhs;
+  a.diag = (const T *)L.diag;
+  a.tm = (T *)L.tm;
+  a.c_np = T(L.c_np);
+  a.c_tm = T(L.c_tm);
+  a.cm = (const T *)L.cell_metric;
+  a.fm = (const T *)L.face_metric;
+  a.ncells = L.ncells;
+  a.nitems = L.diagonal ? L.ncells * N * N * N : L.ncells;
+  a.nx = L.nc[0];
+  a.ny = L.nc[1];
+  a.nz = L.nc[2];
+  constexpr int CB = gen_cells_per_block<N>();
+  const dim3 grid((unsigned)((a.nitems + CB - 1) / CB)), block(gen_threads<N>());
+  if (L.diagonal) hipLaunchKernelGGL((sipg_general_kernel<N, T, false, true>), grid, block, 0, s, a);
+  else if (L.cheby) hipLaunchKernelGGL((sipg_general_kernel<N, T, true, false>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((sipg_general_kernel<N, T, false, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t dispatch_general(const SipgLaunch &L, hipStream_t s)
+{
+  switch (L.tables->n) {
+#define SFK_CASE(NN) \
+  case NN: return launch_general_n<NN, T>(L, s);
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_sipg_general(const SipgLaunch &L, hipStream_t s)
+{
+  return L.fp32 ? dispatch_general<float>(L, s) : dispatch_general<double>(L, s);
+}
+
+// ------------------------------------------------------------------------
+// sine-deformed mesh metric: x = xhat + amp sin(pi xhat) sin(pi yhat)
+// sin(pi zhat) (1,1,1) on the uniform mesh of [0,1]^3 (setup, FP64 math)
+// ------------------------------------------------------------------------
+struct MetricArgs {
+  double xg[16], wg[16];
+  int n, nx, ny, nz;
+  double amp;
+  long long ncells;
+  double *vol;
+  double *area; // [d][cell]
+  void *cm, *fm;
+  int fp32;
+};
+
+__device__ inline void sine_jacobian(const MetricArgs &a, const double xh[3], double J[3][3])
+{
+  constexpr double pi = 3.14159265358979323846;
+  const double sx = sin(pi * xh[0]), sy = sin(pi * xh[1]), sz = sin(pi * xh[2]);
+  const double cx = cos(pi * xh[0]), cy = cos(pi * xh[1]), cz = cos(pi * xh[2]);
+  const double g[3] = {a.amp * pi * cx * sy * sz, a.amp * pi * sx * cy * sz, a.amp * pi * sx * sy * cz};
+  const double h[3] = {1.0 / a.nx, 1.0 / a.ny, 1.0 / a.nz};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) J[i][j] = h[j] * ((i == j ? 1.0 : 0.0) + g[j]);
+}
+
+__device__ inline double inverse3(const double J[3][3], double Ji[3][3])
+{
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  const double id = 1.0 / det;
+  Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  return det;
+}
+
+__device__ inline void store(void *base, long long idx, double v, int fp32)
+{
+  if (fp32) ((float *)base)[idx] = (float)v;
+  else ((double *)base)[idx] = v;
+}
+
+__device__ inline void cell_coords(const MetricArgs &a, long long c, int ci[3])
+{
+  ci[0] = (int)(c % a.nx);
+  const long long r = c / a.nx;
+  ci[1] = (int)(r % a.ny);
+  ci[2] = (int)(r / a.ny);
+}
+
+__global__ void metric_cell_kernel(const MetricArgs a)
+{
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= a.ncells) return;
+  const int n = a.n, n3 = n * n * n;
+  int ci[3];
+  cell_coords(a, c, ci);
+  const double h[3] = {1.0 / a.nx, 1.0 / a.ny, 1.0 / a.nz};
+  double vol = 0.;
+  for (int q = 0; q < n3; ++q) {
+    const int qi[3] = {q % n, (q / n) % n, q / (n * n)};
+    double xh[3], J[3][3], Ji[3][3];
+    for (int d = 0; d < 3; ++d) xh[d] = (ci[d] + a.xg[qi[d]]) * h[d];
+    sine_jacobian(a, xh, J);
+    const double det = inverse3(J, Ji);
+    const double jxw = det * a.wg[qi[0]] * a.wg[qi[1]] * a.wg[qi[2]];
+    vol += jxw;
+    double G[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) G[i][j] = jxw * (Ji[i][0] * Ji[j][0] + Ji[i][1] * Ji[j][1] + Ji[i][2] * Ji[j][2]);
+    const long long b = c * 6 * n3 + q;
+    store(a.cm, b, G[0][0], a.fp32);
+    store(a.cm, b + n3, G[1][1], a.fp32);
+    store(a.cm, b + 2 * n3, G[2][2], a.fp32);
+    store(a.cm, b + 3 * n3, G[0][1], a.fp32);
+    store(a.cm, b + 4 * n3, G[0][2], a.fp32);
+    store(a.cm, b + 5 * n3, G[1][2], a.fp32);
+  }
+  if (a.vol) a.vol[c] = vol;
+}
+
+// pass 1 (write = 0): face areas; pass 2 (write = 1): face data with tau
+__global__ void metric_face_kernel(const MetricArgs a, int write)
+{
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= 3 * a.ncells) return;
+  const int d = (int)(id / a.ncells);
+  const long long c = id % a.ncells;
+  const int n = a.n, n2 = n * n, p = n - 1;
+  int ci[3], cu[3];
+  cell_coords(a, c, ci);
+  for (int e = 0; e < 3; ++e) cu[e] = ci[e];
+  const int nd = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
+  cu[d] = (cu[d] + 1) % nd;
+  const long long u = ((long long)cu[2] * a.ny + cu[1]) * a.nx + cu[0];
+  const int lo = d == 0 ? 1 : 0, hi = d == 2 ? 1 : 2;
+  const double h[3] = {1.0 / a.nx, 1.0 / a.ny, 1.0 / a.nz};
+  double tau = 0.;
+  if (write) {
+    const double A = a.area[(long long)d * a.ncells + c];
+    tau = double(p + 1) * double(p + 1) * 0.5 * (A / a.vol[c] + A / a.vol[u]);
+  }
+  double area = 0.;
+  for (int fa = 0; fa < n2; ++fa) {
+    double xiL[3], xiU[3];
+    xiL[lo] = xiU[lo] = a.xg[fa % n];
+    xiL[hi] = xiU[hi] = a.xg[fa / n];
+    xiL[d] = 1.0;
+    xiU[d] = 0.0;
+    double xL[3], xU[3], JL[3][3], JU[3][3], JLi[3][3], JUi[3][3];
+    for (int e = 0; e < 3; ++e) {
+      xL[e] = (ci[e] + xiL[e]) * h[e];
+      xU[e] = (cu[e] + xiU[e]) * h[e];
+    }
+    sine_jacobian(a, xL, JL);
+    sine_jacobian(a, xU, JU);
+    const double detL = inverse3(JL, JLi);
+    inverse3(JU, JUi);
+    double nv[3] = {JLi[d][0], JLi[d][1], JLi[d][2]}; // J^{-T} e_d
+    const double nrm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    const double wf = a.wg[fa % n] * a.wg[fa / n];
+    const double jxwf = detL * nrm * wf;
+    area += jxwf;
+    if (!write) continue;
+    for (int e = 0; e < 3; ++e) nv[e] /= nrm;
+    const long long b = ((long long)d * a.ncells + c) * 8 * n2 + fa;
+    for (int e = 0; e < 3; ++e) {
+      store(a.fm, b + e * n2, jxwf * (JLi[e][0] * nv[0] + JLi[e][1] * nv[1] + JLi[e][2] * nv[2]), a.fp32);
+      store(a.fm, b + (3 + e) * n2, jxwf * (JUi[e][0] * nv[0] + JUi[e][1] * nv[1] + JUi[e][2] * nv[2]), a.fp32);
+    }
+    store(a.fm, b + 6 * n2, jxwf, a.fp32);
+    store(a.fm, b + 7 * n2, tau * jxwf, a.fp32);
+  }
+  if (!write) a.area[(long long)d * a.ncells + c] = area;
+}
+
+hipError_t launch_sine_metric(const Tables1D &t, bool fp32, const int nc[3], double amp, void *cell_metric,
+                              void *face_metric, hipStream_t s)
+{
+  MetricArgs a{};
+  a.n = t.n;
+  for (int i = 0; i < t.n; ++i) {
+    a.xg[i] = t.gauss[i];
+    a.wg[i] = t.weights[i];
+  }
+  a.nx = nc[0];
+  a.ny = nc[1];
+  a.nz = nc[2];
+  a.amp = amp;
+  a.ncells = (long long)nc[0] * nc[1] * nc[2];
+  a.cm = cell_metric;
+  a.fm = face_metric;
+  a.fp32 = fp32 ? 1 : 0;
+  hipError_t e = hipMalloc(&a.vol, sizeof(double) * a.ncells * 4);
+  if (e != hipSuccess) return e;
+  a.area = a.vol + a.ncells;
+  const int bs = 128;
+  hipLaunchKernelGGL(metric_cell_kernel, dim3((unsigned)((a.ncells + bs - 1) / bs)), dim3(bs), 0, s, a);
+  if (face_metric) {
+    const unsigned g = (unsigned)((3 * a.ncells + bs - 1) / bs);
+    hipLaunchKernelGGL(metric_face_kernel, dim3(g), dim3(bs), 0, s, a, 0);
+    hipLaunchKernelGGL(metric_face_kernel, dim3(g), dim3(bs), 0, s, a, 1);
+  }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(a.vol);
+  return e;
+}
+
+} // namespace sfk

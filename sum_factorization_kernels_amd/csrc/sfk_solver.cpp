@@ -1,0 +1,347 @@
+// Chebyshev-preconditioned CG for the continuous Q_p Laplacian, all on the
+// device (the reference has no solver; north-star extension).
+//
+//   FP64 smoother:  PCG in FP64 with a Chebyshev(k) point-Jacobi polynomial
+//                   preconditioner, to ||b - A x|| <= tol ||b||.
+//   FP32 smoother:  defect correction in FP64 (r = b - A x with the FP64
+//                   operator, x += e) around an inner flexible PCG that runs
+//                   entirely on the FP32 operator with FP32 vectors (inner
+//                   products accumulated in FP64) and the FP32 Chebyshev
+//                   preconditioner; each inner solve reduces its residual by
+//                   kInnerReduction.
+// Chebyshev bounds: lambda_max of D^{-1}A from `lanczos_steps` FP64 Lanczos
+// steps on D^{-1/2} A D^{-1/2} (largest Ritz value by bisection on the
+// tridiagonal matrix), times 1.1; lambda_min = lambda_max / 30.
+// The Chebyshev recurrence (Saad, Iterative Methods for Sparse Linear
+// Systems, Alg. 12.1) is written in the fused update form of
+// EvaluationDGLaplacian::do_cheby_iter (legacy/include/evaluation_dg_laplacian.h
+// :485-502): tmp = c_tm*tmp + c_np*D^{-1}(A z - r), z = z - tmp.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "sfk.h"
+#include "sfk_internal.h"
+#include "sfk_kernels.h"
+#include "sfk_op.h"
+
+namespace sfk {
+namespace {
+
+constexpr double kInnerReduction = 1e-4;
+constexpr int kPartials = 2048;
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+};
+
+__global__ void inv_diag_kernel(const double *d, double *inv, float *inv32, double *isq, long long n)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double v = d[i];
+    const double iv = v != 0. ? 1. / v : 0.;
+    inv[i] = iv;
+    if (inv32) inv32[i] = (float)iv;
+    isq[i] = v != 0. ? 1. / sqrt(v) : 0.;
+  }
+}
+
+__global__ void scale_kernel(const double *a, const double *b, double *c, long long n)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    c[i] = a[i] * b[i];
+}
+
+// deterministic start vector for Lanczos (interior only: the inverse
+// diagonal square root is 0 on Dirichlet rows)
+__global__ void start_vector_kernel(const double *isq, double *v, long long n)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    unsigned long long h = (unsigned long long)i * 0x9E3779B97F4A7C15ULL;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ULL;
+    h ^= h >> 32;
+    v[i] = isq[i] != 0. ? (double)(h & 0xFFFFFF) / double(0xFFFFFF) - 0.5 : 0.;
+  }
+}
+
+unsigned grid_of(long long n) { return (unsigned)std::min<long long>((n + 255) / 256, 65535LL * 8); }
+
+// largest eigenvalue of the symmetric tridiagonal (alpha, beta) by bisection
+double tridiag_max_eig(const std::vector<double> &al, const std::vector<double> &be)
+{
+  const int m = (int)al.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? std::fabs(be[i - 1]) : 0.) + (i + 1 < m ? std::fabs(be[i]) : 0.);
+    lo = std::min(lo, al[i] - r);
+    hi = std::max(hi, al[i] + r);
+  }
+  auto count_below = [&](double x) { // Sturm count of eigenvalues < x
+    int cnt = 0;
+    double q = 1.;
+    for (int i = 0; i < m; ++i) {
+      q = al[i] - x - (i > 0 ? be[i - 1] * be[i - 1] / q : 0.);
+      if (q == 0.) q = 1e-300;
+      if (q < 0.) ++cnt;
+    }
+    return cnt;
+  };
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) < m) lo = mid;
+    else hi = mid;
+  }
+  return hi;
+}
+
+struct Solver {
+  sfk_op *op, *sm;
+  hipStream_t s;
+  long long n;
+  bool fp32;
+  double *partials = nullptr, *dres = nullptr;
+  sfk_solve_stats st{};
+
+  sfk_status dot(bool f32, const void *x, const void *y, double &out)
+  {
+    hipError_t e = vec_dot(f32, x, y, n, dres, partials, kPartials, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&out, dres, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error(std::string("solver: ") + hipGetErrorString(e));
+      return SFK_ERR_HIP;
+    }
+    if (!std::isfinite(out)) {
+      set_error("solver: non-finite inner product");
+      return SFK_ERR_NONFINITE;
+    }
+    return SFK_OK;
+  }
+};
+
+#define SOLVER_HIP(expr)                                              \
+  do {                                                                \
+    hipError_t e_ = (expr);                                           \
+    if (e_ != hipSuccess) {                                           \
+      set_error(std::string("solver: ") + hipGetErrorString(e_));     \
+      return SFK_ERR_HIP;                                             \
+    }                                                                 \
+  } while (0)
+#define SOLVER_ST(expr)          \
+  do {                           \
+    sfk_status s_ = (expr);      \
+    if (s_ != SFK_OK) return s_; \
+  } while (0)
+
+} // namespace
+} // namespace sfk
+
+using namespace sfk;
+
+extern "C" sfk_status sfk_op_rhs_constant(sfk_op *op, double f, void *b, void *stream);
+
+extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const double *b, double *x, double rel_tol,
+                                         int max_iter, int cheby_degree, int lanczos_steps, sfk_solve_stats *stats,
+                                         void *stream)
+{
+  if (!op || !smoother_op || !b || !x || !stats) {
+    set_error("sfk_solve_cg_cheby: null argument");
+    return SFK_ERR_INVALID_ARG;
+  }
+  if (op->d.kind != SFK_CONT_LAPLACE || smoother_op->d.kind != SFK_CONT_LAPLACE || op->d.precision != SFK_FP64 ||
+      op->d.n_rhs != 1 || smoother_op->d.n_rhs != 1 || op->nvec != smoother_op->nvec ||
+      op->d.degree != smoother_op->d.degree) {
+    set_error("sfk_solve_cg_cheby: op must be an FP64 continuous Laplacian and smoother_op the same mesh");
+    return SFK_ERR_INVALID_ARG;
+  }
+  if (!(rel_tol > 0) || max_iter < 1 || cheby_degree < 1 || cheby_degree > 32 || lanczos_steps < 2 ||
+      lanczos_steps > 200) {
+    set_error("sfk_solve_cg_cheby: invalid tolerance, iteration count, degree or Lanczos steps");
+    return SFK_ERR_INVALID_ARG;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  Solver S{op, smoother_op, (hipStream_t)stream, op->nvec, smoother_op->d.precision == SFK_FP32};
+  const long long n = S.n;
+  const bool f32 = S.fp32;
+  const size_t n8 = (size_t)n * 8, ns = (size_t)n * (f32 ? 4 : 8);
+  // device work space: FP64 vectors, and smoother-precision vectors
+  DevBuf red, d64, inv64, isq, r, w, lv0, lv1, lw, lt, e64;
+  DevBuf rs, es, zs, ps, qs, zero, tms, rold, inv_s;
+  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (kPartials + 1)));
+  S.partials = (double *)red.p;
+  S.dres = S.partials + kPartials;
+  for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
+    SOLVER_HIP(hipMalloc(&bb->p, n8));
+  for (DevBuf *bb : {&rs, &es, &zs, &ps, &qs, &zero, &tms, &rold, &inv_s})
+    SOLVER_HIP(hipMalloc(&bb->p, ns));
+  const hipStream_t s = S.s;
+  SOLVER_HIP(vec_fill(f32, zero.p, n, 0., s));
+  // point-Jacobi diagonal (FP64), inverse in both precisions
+  SOLVER_ST(sfk_compute_diagonal(op, d64.p, stream));
+  hipLaunchKernelGGL(inv_diag_kernel, dim3(grid_of(n)), dim3(256), 0, s, (const double *)d64.p, (double *)inv64.p,
+                     f32 ? (float *)inv_s.p : nullptr, (double *)isq.p, n);
+  SOLVER_HIP(hipGetLastError());
+  const void *inv_sm = f32 ? inv_s.p : inv64.p;
+
+  // ---- Lanczos on D^{-1/2} A D^{-1/2} (FP64)
+  hipLaunchKernelGGL(start_vector_kernel, dim3(grid_of(n)), dim3(256), 0, s, (const double *)isq.p,
+                     (double *)lv1.p, n);
+  double nrm2;
+  SOLVER_ST(S.dot(false, lv1.p, lv1.p, nrm2));
+  SOLVER_HIP(vec_axpby(false, n, 0., lv1.p, 1. / std::sqrt(nrm2), lv1.p, s));
+  SOLVER_HIP(vec_fill(false, lv0.p, n, 0., s));
+  std::vector<double> al, be;
+  double beta = 0.;
+  for (int k = 0; k < lanczos_steps; ++k) {
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_of(n)), dim3(256), 0, s, (const double *)lv1.p, (const double *)isq.p,
+                       (double *)lt.p, n);
+    SOLVER_ST(sfk_apply(op, lt.p, lw.p, stream));
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_of(n)), dim3(256), 0, s, (const double *)lw.p, (const double *)isq.p,
+                       (double *)lw.p, n);
+    ++S.st.matvecs_outer;
+    double alpha;
+    SOLVER_ST(S.dot(false, lw.p, lv1.p, alpha));
+    SOLVER_HIP(vec_axpby(false, n, -alpha, lv1.p, 1., lw.p, s));
+    SOLVER_HIP(vec_axpby(false, n, -beta, lv0.p, 1., lw.p, s));
+    al.push_back(alpha);
+    SOLVER_ST(S.dot(false, lw.p, lw.p, nrm2));
+    beta = std::sqrt(nrm2);
+    if (k + 1 == lanczos_steps || beta < 1e-14 * std::fabs(alpha)) break;
+    be.push_back(beta);
+    std::swap(lv0.p, lv1.p);
+    SOLVER_HIP(vec_axpby(false, n, 1. / beta, lw.p, 0., lv1.p, s));
+  }
+  const double lmax = 1.1 * tridiag_max_eig(al, be);
+  const double lmin = lmax / 30.;
+  S.st.lambda_max = lmax;
+  S.st.lambda_min = lmin;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma1 = theta / delta;
+
+  // z = P(rr) on the smoother operator; *z_io and *spare are two distinct
+  // buffers, the result ends in *z_io (pointers swapped as needed)
+  auto cheby = [&](const void *rr, void **z_io, void **spare) -> sfk_status {
+    void *z = *z_io;
+    // step 0: tm = (1/theta) D^{-1}(0 - rr), z = 0 - tm
+    SOLVER_HIP(vec_cheby_update(f32, n, zero.p, rr, zero.p, inv_sm, z, tms.p, 1. / theta, 0., s));
+    double rho_old = 1. / sigma1;
+    for (int j = 1; j < cheby_degree; ++j) {
+      const double rho = 1. / (2. * sigma1 - rho_old);
+      SOLVER_ST(sfk_cheby_step(S.sm, rr, z, inv_sm, *spare, tms.p, 2. * rho / delta, rho * rho_old, stream));
+      std::swap(z, *spare);
+      rho_old = rho;
+      ++S.st.matvecs_inner;
+    }
+    *z_io = z;
+    return SFK_OK;
+  };
+
+  // ---- residual of the initial guess (FP64)
+  double bb;
+  SOLVER_ST(S.dot(false, b, b, bb));
+  const double bnorm = std::sqrt(bb);
+  if (bnorm == 0.) {
+    SOLVER_HIP(vec_fill(false, x, n, 0., s));
+    *stats = S.st;
+    return SFK_OK;
+  }
+  SOLVER_ST(sfk_apply(op, x, r.p, stream));
+  ++S.st.matvecs_outer;
+  SOLVER_HIP(vec_axpby(false, n, 1., b, -1., r.p, s));
+  double rr;
+  SOLVER_ST(S.dot(false, r.p, r.p, rr));
+  S.st.initial_residual = std::sqrt(rr);
+  double res = S.st.initial_residual;
+  int iters = 0;
+
+  // flexible PCG for A e = r (r = the FP64 residual in r.p), e = 0 at start,
+  // until ||r - A e|| <= target (absolute) or `budget` iterations; runs in
+  // the smoother's precision on the smoother operator (FP64 case: op)
+  auto pcg = [&](double target, int budget, int &used) -> sfk_status {
+    const size_t vb = ns;
+    void *rv = f32 ? rs.p : r.p;
+    if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rv, n, s));
+    void *e = f32 ? es.p : w.p;
+    SOLVER_HIP(vec_fill(f32, e, n, 0., s));
+    void *z = zs.p, *spare = qs.p;
+    SOLVER_ST(cheby(rv, &z, &spare));
+    SOLVER_HIP(hipMemcpyAsync(ps.p, z, vb, hipMemcpyDeviceToDevice, s));
+    double rz;
+    SOLVER_ST(S.dot(f32, rv, z, rz));
+    used = 0;
+    while (used < budget) {
+      void *qv = spare; // neither z nor p
+      SOLVER_ST(sfk_apply(S.sm, ps.p, qv, stream));
+      if (f32) ++S.st.matvecs_inner;
+      else ++S.st.matvecs_outer;
+      double pq;
+      SOLVER_ST(S.dot(f32, ps.p, qv, pq));
+      const double alpha = rz / pq;
+      SOLVER_HIP(vec_axpby(f32, n, alpha, ps.p, 1., e, s));
+      SOLVER_HIP(hipMemcpyAsync(rold.p, rv, vb, hipMemcpyDeviceToDevice, s));
+      SOLVER_HIP(vec_axpby(f32, n, -alpha, qv, 1., rv, s));
+      ++used;
+      double rn;
+      SOLVER_ST(S.dot(f32, rv, rv, rn));
+      if (std::sqrt(rn) <= target) break;
+      SOLVER_ST(cheby(rv, &z, &spare));
+      double zr, zro;
+      SOLVER_ST(S.dot(f32, z, rv, zr));
+      SOLVER_ST(S.dot(f32, z, rold.p, zro));
+      const double beta_cg = (zr - zro) / rz; // flexible (Polak-Ribiere) coefficient
+      rz = zr;
+      SOLVER_HIP(vec_axpby(f32, n, 1., z, beta_cg, ps.p, s));
+    }
+    return SFK_OK;
+  };
+
+  const double target = rel_tol * bnorm;
+  if (!f32) {
+    // all FP64: one PCG to the final tolerance
+    if (res > target) {
+      int used = 0;
+      SOLVER_ST(pcg(target, max_iter, used));
+      iters = used;
+      SOLVER_HIP(vec_axpby(false, n, 1., w.p, 1., x, s));
+    }
+  } else {
+    // defect correction: FP64 residual and update, FP32 inner PCG
+    int outer = 0;
+    while (res > target && iters < max_iter && outer < 60) {
+      int used = 0;
+      SOLVER_ST(pcg(std::max(kInnerReduction * res, 0.5 * target), max_iter - iters, used));
+      iters += used;
+      ++outer;
+      SOLVER_HIP(vec_copy_convert(false, es.p, e64.p, n, s));
+      SOLVER_HIP(vec_axpby(false, n, 1., e64.p, 1., x, s));
+      SOLVER_ST(sfk_apply(op, x, r.p, stream));
+      ++S.st.matvecs_outer;
+      SOLVER_HIP(vec_axpby(false, n, 1., b, -1., r.p, s));
+      SOLVER_ST(S.dot(false, r.p, r.p, rr));
+      res = std::sqrt(rr);
+    }
+  }
+  // final FP64 residual
+  SOLVER_ST(sfk_apply(op, x, r.p, stream));
+  ++S.st.matvecs_outer;
+  SOLVER_HIP(vec_axpby(false, n, 1., b, -1., r.p, s));
+  SOLVER_ST(S.dot(false, r.p, r.p, rr));
+  SOLVER_HIP(hipStreamSynchronize(s));
+  S.st.final_residual = std::sqrt(rr);
+  S.st.iterations = iters;
+  S.st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *stats = S.st;
+  if (!(S.st.final_residual <= target)) {
+    set_error("sfk_solve_cg_cheby: no convergence (relative residual " +
+              std::to_string(S.st.final_residual / bnorm) + ")");
+    return SFK_ERR_NO_CONVERGENCE;
+  }
+  return SFK_OK;
+}
