@@ -195,10 +195,25 @@ sfk_status sfk_layout_to_canonical(int degree, const int n_cells[3], int lanes, 
 sfk_status sfk_layout_to_interleaved(int degree, const int n_cells[3], int lanes, const double *in,
                                      double *out, void *stream);
 
+/* ---- device memory (for C/C++ callers without a HIP toolchain) ---- */
+sfk_status sfk_vec_alloc(size_t bytes, void **ptr);
+sfk_status sfk_vec_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device (synchronous) */
+sfk_status sfk_memcpy(void *dst, const void *src, size_t bytes, int kind);
+sfk_status sfk_synchronize(void *stream);
+
 /* ---- vector helpers on device arrays (FP64 / FP32 per `precision`) ---- */
 sfk_status sfk_vec_fill(int precision, void *x, long long n, double value, void *stream);
 sfk_status sfk_vec_dot(int precision, const void *x, const void *y, long long n, double *result,
                        void *stream);
+/* unfused Chebyshev vector update (EvaluationDGLaplacian::
+ * emulate_cheby_vector_updates, evaluation_dg_laplacian.h:556-601, with
+ * au = A sol_old supplied by the caller):
+ *   tmp = c_tm*tmp + c_np*diag*(au - rhs);  sol_new = sol_old - tmp
+ * au may alias sol_new (elementwise). */
+sfk_status sfk_vec_cheby_update(int precision, long long n, const void *au, const void *rhs,
+                                const void *sol_old, const void *diag, void *sol_new, void *tmp,
+                                double c_np, double c_tm, void *stream);
 
 const char *sfk_last_error(void);
 const char *sfk_version(void);

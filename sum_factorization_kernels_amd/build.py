@@ -56,7 +56,24 @@ def build(jobs=None, verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_drivers()
     return LIB
+
+
+DRIVERS = os.path.join(PKG, "drivers")
+
+
+def build_drivers():
+    """C++ drivers (reference report formats) against the C ABI, plain g++"""
+    for src in sorted(glob.glob(os.path.join(DRIVERS, "*.cc"))):
+        exe = os.path.join(DRIVERS, os.path.splitext(os.path.basename(src))[0])
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+               "-I" + os.path.join(PKG, "include"), src, "-o", exe, "-L" + PKG, "-lsfk",
+               "-Wl,-rpath," + PKG]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("driver build failed: " + src)
 
 
 if __name__ == "__main__":

@@ -552,6 +552,51 @@ sfk_status sfk_layout_to_interleaved(int degree, const int n_cells[3], int lanes
   return SFK_OK;
 }
 
+sfk_status sfk_vec_cheby_update(int precision, long long n, const void *au, const void *rhs, const void *sol_old,
+                                const void *diag, void *sol_new, void *tmp, double c_np, double c_tm, void *stream)
+{
+  SFK_REQUIRE(au && rhs && sol_old && diag && sol_new && tmp && n >= 0, SFK_ERR_INVALID_ARG,
+              "sfk_vec_cheby_update: invalid argument");
+  SFK_CHECK_HIP(vec_cheby_update(precision == SFK_FP32, n, au, rhs, sol_old, diag, sol_new, tmp, c_np, c_tm,
+                                 (hipStream_t)stream));
+  return SFK_OK;
+}
+
+sfk_status sfk_vec_alloc(size_t bytes, void **ptr)
+{
+  SFK_REQUIRE(ptr, SFK_ERR_INVALID_ARG, "sfk_vec_alloc: null argument");
+  *ptr = nullptr;
+  if (bytes == 0) return SFK_OK;
+  if (hipMalloc(ptr, bytes) != hipSuccess) {
+    *ptr = nullptr;
+    set_error("sfk_vec_alloc: hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    return SFK_ERR_ALLOC;
+  }
+  return SFK_OK;
+}
+
+sfk_status sfk_vec_free(void *ptr)
+{
+  if (ptr) SFK_CHECK_HIP(hipFree(ptr));
+  return SFK_OK;
+}
+
+sfk_status sfk_memcpy(void *dst, const void *src, size_t bytes, int kind)
+{
+  SFK_REQUIRE((dst && src) || bytes == 0, SFK_ERR_INVALID_ARG, "sfk_memcpy: null argument");
+  SFK_REQUIRE(kind >= 0 && kind <= 2, SFK_ERR_INVALID_ARG, "sfk_memcpy: kind must be 0, 1 or 2");
+  if (bytes == 0) return SFK_OK;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  SFK_CHECK_HIP(hipMemcpy(dst, src, bytes, k));
+  return SFK_OK;
+}
+
+sfk_status sfk_synchronize(void *stream)
+{
+  SFK_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return SFK_OK;
+}
+
 sfk_status sfk_vec_fill(int precision, void *x, long long n, double value, void *stream)
 {
   SFK_REQUIRE(x && n >= 0, SFK_ERR_INVALID_ARG, "invalid argument");

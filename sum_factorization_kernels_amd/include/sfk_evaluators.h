@@ -1,0 +1,478 @@
+// C++ evaluator classes with the reference's names and methods, running on
+// the MI355X through the C ABI of include/sfk.h (no HIP types here: plain
+// g++ can compile code that uses this header; link with libsfk.so).
+//
+//   EvaluationCellLaplacian<dim,degree,Number>  legacy/include/evaluation_cell_laplacian.h:26-908
+//   EvaluationDGLaplacian<dim,degree,Number>    legacy/include/evaluation_dg_laplacian.h:33-716
+//   EvaluationDGAdvection<dim,degree,Number>    legacy/include/evaluation_dg_advect.h:96-845
+//
+// As in the reference, an evaluator owns its vectors (here in device memory,
+// canonical cell-major layout) and initialize() sets the initial data.  Only
+// dim == 3 exists.  Differences in behaviour, all deliberate:
+//  * errors throw sfk::Error (status code + message) instead of calling
+//    std::terminate / abort;
+//  * x-direction periodic neighbours are exact for every cell (the
+//    reference's evaluation_dg_laplacian.h:270 reads the last cell's +x
+//    neighbour from a wrong address);
+//  * the SIPG operator applies S, not S^T, in its last x sweep
+//    (evaluation_dg_laplacian.h:481-483); set ref_x_sweep = true before
+//    initialize() to reproduce the reference's numbers instead;
+//  * cells per x row need not be a multiple of a SIMD width (lanes = 1 on
+//    the GPU), and block sizes blx/bly/blz are accepted but do not change
+//    the GPU schedule.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sfk.h"
+
+namespace sfk {
+
+struct Error : std::runtime_error {
+  sfk_status status;
+  Error(sfk_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+inline void check(sfk_status s)
+{
+  if (s != SFK_OK) throw Error(s, sfk_last_error());
+}
+
+template <typename Number>
+constexpr int precision_of()
+{
+  return sizeof(Number) == 4 ? SFK_FP32 : SFK_FP64;
+}
+
+// device vector of Number (replaces AlignedVector, legacy/include/aligned_vector.h)
+template <typename Number>
+class DeviceVector {
+public:
+  DeviceVector() = default;
+  explicit DeviceVector(std::size_t n) { resize(n); }
+  DeviceVector(const DeviceVector &) = delete;
+  DeviceVector &operator=(const DeviceVector &) = delete;
+  DeviceVector(DeviceVector &&o) noexcept : ptr_(o.ptr_), n_(o.n_) { o.ptr_ = nullptr, o.n_ = 0; }
+  ~DeviceVector() { sfk_vec_free(ptr_); }
+  void resize(std::size_t n)
+  {
+    check(sfk_vec_free(ptr_));
+    ptr_ = nullptr;
+    check(sfk_vec_alloc(n * sizeof(Number), &ptr_));
+    n_ = n;
+  }
+  std::size_t size() const { return n_; }
+  void *data() { return ptr_; }
+  const void *data() const { return ptr_; }
+  void fill(double v) { check(sfk_vec_fill(precision_of<Number>(), ptr_, (long long)n_, v, nullptr)); }
+  void upload(const std::vector<Number> &h)
+  {
+    if (h.size() != n_) throw Error(SFK_ERR_INVALID_ARG, "DeviceVector::upload: size mismatch");
+    check(sfk_memcpy(ptr_, h.data(), n_ * sizeof(Number), 0));
+  }
+  std::vector<Number> download() const
+  {
+    std::vector<Number> h(n_);
+    check(sfk_memcpy(h.data(), ptr_, n_ * sizeof(Number), 1));
+    return h;
+  }
+
+private:
+  void *ptr_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+class Operator {
+public:
+  explicit Operator(const sfk_op_desc &d) { check(sfk_op_create(&d, &op_)); }
+  Operator(const Operator &) = delete;
+  Operator &operator=(const Operator &) = delete;
+  ~Operator() { sfk_op_destroy(op_); }
+  sfk_op *get() const { return op_; }
+  long long vector_size() const
+  {
+    long long n = 0;
+    check(sfk_op_vector_size(op_, &n));
+    return n;
+  }
+
+private:
+  sfk_op *op_ = nullptr;
+};
+
+inline sfk_op_desc make_desc(int kind, int degree, int precision, int basis, const unsigned *n_cells,
+                             const double *inv_jac)
+{
+  sfk_op_desc d{};
+  d.kind = kind;
+  d.degree = degree;
+  d.precision = precision;
+  d.basis = basis;
+  d.geometry = SFK_GEOM_CARTESIAN;
+  d.flags = 0;
+  for (int k = 0; k < 3; ++k) {
+    d.n_cells[k] = (int)n_cells[k];
+    d.inv_jac[k] = inv_jac[k];
+  }
+  d.n_rhs = 1;
+  return d;
+}
+
+// inverse Jacobian diagonal of the reference drivers (evaluation_cell_laplacian.h
+// :803-813, evaluation_dg_laplacian.h:605-613)
+inline std::vector<double> reference_diagonal_jacobian() { return {4., 2., 4. / 3.}; }
+
+} // namespace sfk
+
+// --------------------------------------------------------------------------
+template <int dim, int degree, typename Number>
+class EvaluationCellLaplacian {
+  static_assert(dim == 3, "three-dimensional operators only");
+
+public:
+  static constexpr unsigned int dimension = dim;
+  static constexpr unsigned int n_q_points = (degree + 1) * (degree + 1) * (degree + 1);
+  static constexpr unsigned int dofs_per_cell = n_q_points;
+  int basis = SFK_BASIS_HERMITE; // the reference class uses the Hermite-like basis (:822)
+
+  // n_element_batches = number of cells (one cell per GPU "lane")
+  void initialize(const unsigned int n_element_batches, const bool is_cartesian)
+  {
+    if (!is_cartesian)
+      throw sfk::Error(SFK_ERR_UNSUPPORTED, "EvaluationCellLaplacian: Cartesian cells only (the reference aborts, :81)");
+    const unsigned nc[3] = {n_element_batches, 1, 1};
+    const std::vector<double> ij = sfk::reference_diagonal_jacobian();
+    op_.reset(new sfk::Operator(
+        sfk::make_desc(SFK_CELL_LAPLACE, degree, sfk::precision_of<Number>(), basis, nc, ij.data())));
+    n_cells_ = n_element_batches;
+    input_array.resize(std::size_t(n_cells_) * dofs_per_cell);
+    output_array.resize(std::size_t(n_cells_) * dofs_per_cell);
+    input_array.fill(0.);
+    output_array.fill(0.);
+  }
+  std::size_t n_elements() const { return n_cells_; }
+
+  // boundary-integral check of evaluation_cell_laplacian.h:108-214 for the
+  // basis in use: A_K x_d must equal the boundary integral of the linear
+  // function; returns the largest error over the three directions.
+  double do_verification();
+
+  void matrix_vector_product()
+  {
+    sfk::check(sfk_apply(op_->get(), input_array.data(), output_array.data(), nullptr));
+    sfk::check(sfk_synchronize(nullptr));
+  }
+
+  sfk::DeviceVector<Number> input_array, output_array;
+
+private:
+  std::unique_ptr<sfk::Operator> op_;
+  unsigned n_cells_ = 0;
+};
+
+// --------------------------------------------------------------------------
+template <int dim, int degree, typename Number>
+class EvaluationDGLaplacian {
+  static_assert(dim == 3, "three-dimensional operators only");
+
+public:
+  static constexpr unsigned int dimension = dim;
+  static constexpr unsigned int n_q_points = (degree + 1) * (degree + 1) * (degree + 1);
+  static constexpr unsigned int dofs_per_cell = n_q_points;
+  unsigned int blx = 1, bly = 1, blz = 1;
+  bool ref_x_sweep = false;
+
+  // n_cells_in[0] counts cells (the reference divides it by its SIMD width)
+  void initialize(const unsigned int *n_cells_in)
+  {
+    for (int d = 0; d < 3; ++d) n_cells[d] = n_cells_in[d];
+    const std::vector<double> ij = sfk::reference_diagonal_jacobian();
+    sfk_op_desc d = sfk::make_desc(SFK_DG_SIPG, degree, sfk::precision_of<Number>(), SFK_BASIS_HERMITE, n_cells,
+                                   ij.data());
+    if (ref_x_sweep) d.flags |= SFK_FLAG_REF_X_SWEEP;
+    op_.reset(new sfk::Operator(d));
+    const std::size_t n = n_elements() * dofs_per_cell;
+    for (auto *v : {&sol_old, &sol_new, &mat_diagonal, &sol_tmp, &sol_rhs}) v->resize(n);
+    // initial data of evaluation_dg_laplacian.h:86-90
+    sol_old.fill(1.);
+    sol_new.fill(0.);
+    mat_diagonal.fill(1.);
+    sol_tmp.fill(0.);
+    sol_rhs.fill(1.);
+  }
+  std::size_t n_elements() const { return std::size_t(n_cells[0]) * n_cells[1] * n_cells[2]; }
+  void do_verification() {}
+  void do_matvec() { do_cheby_iter<false>(sol_new, sol_new, sol_new, sol_tmp, sol_tmp, 0., 0.); }
+  void do_chebyshev() { do_cheby_iter<true>(sol_rhs, sol_old, mat_diagonal, sol_new, sol_tmp, 0.5, 0.5); }
+
+  // evaluate_chebyshev = false: vec_tm... = A sol_old written to sol_new
+  // (do_matvec passes sol_tmp); true: the fused update of :485-502 with
+  // `src` as the right-hand side
+  template <bool evaluate_chebyshev = true>
+  void do_cheby_iter(const sfk::DeviceVector<Number> &src, const sfk::DeviceVector<Number> &sol_old_in,
+                     const sfk::DeviceVector<Number> &diag, sfk::DeviceVector<Number> &sol_new_out,
+                     sfk::DeviceVector<Number> &vec_tm, const Number coefficient_np, const Number coefficient_tm)
+  {
+    if (evaluate_chebyshev)
+      sfk::check(sfk_cheby_step(op_->get(), src.data(), sol_old_in.data(), diag.data(), sol_new_out.data(),
+                                vec_tm.data(), coefficient_np, coefficient_tm, nullptr));
+    else
+      sfk::check(sfk_apply(op_->get(), sol_old_in.data(), sol_new_out.data() == sol_old_in.data()
+                                                              ? vec_tm.data()
+                                                              : sol_new_out.data(),
+                           nullptr));
+    sfk::check(sfk_synchronize(nullptr));
+  }
+
+  // the unfused vector update of evaluation_dg_laplacian.h:556-601
+  void emulate_cheby_vector_updates();
+
+  unsigned int n_cells[3] = {1, 1, 1};
+  sfk::DeviceVector<Number> sol_old, sol_new, sol_rhs, sol_tmp, mat_diagonal;
+
+private:
+  std::unique_ptr<sfk::Operator> op_;
+};
+
+// --------------------------------------------------------------------------
+template <int dim>
+struct ExactSolution {
+  // evaluation_dg_advect.h:37-92: sin(4 pi x') cos(4 pi y') cos(4 pi z'),
+  // x' = x - t a, a = (1, 0.15, -0.05)
+  double time, a[3] = {1., 0.15, -0.05};
+  explicit ExactSolution(double t = 0.) : time(t) {}
+  double value(const double *p) const
+  {
+    const double k = 4. * 3.14159265358979323846;
+    return std::sin(k * (p[0] - time * a[0])) * std::cos(k * (p[1] - time * a[1])) *
+           std::cos(k * (p[2] - time * a[2]));
+  }
+  double time_derivative(const double *p) const
+  {
+    const double k = 4. * 3.14159265358979323846;
+    const double x = p[0] - time * a[0], y = p[1] - time * a[1], z = p[2] - time * a[2];
+    const double r = -a[0] * k * std::cos(k * x) * std::cos(k * y) * std::cos(k * z);
+    return r + k * std::sin(k * x) * (a[1] * std::sin(k * y) * std::cos(k * z) + a[2] * std::cos(k * y) * std::sin(k * z));
+  }
+};
+
+template <int dim, int degree, typename Number>
+class EvaluationDGAdvection {
+  static_assert(dim == 3, "three-dimensional operators only");
+
+public:
+  static constexpr unsigned int dimension = dim;
+  static constexpr unsigned int n_q_points = (degree + 1) * (degree + 1) * (degree + 1);
+  static constexpr unsigned int dofs_per_cell = n_q_points;
+  unsigned int blx = 1, bly = 1, blz = 1;
+
+  // domain [0,2]^3 (evaluation_dg_advect.h:719-727); sol_new = exact
+  // solution at the GLL nodes, time step 0.2 h / |a| / p (:160-165)
+  void initialize(const unsigned int *n_cells_in)
+  {
+    for (int d = 0; d < 3; ++d) {
+      n_cells[d] = n_cells_in[d];
+      inv_jac[d] = n_cells[d] / 2.;
+    }
+    sfk_op_desc d = sfk::make_desc(SFK_DG_ADVECT, degree, sfk::precision_of<Number>(), SFK_BASIS_GLL, n_cells, inv_jac);
+    ExactSolution<dim> ex;
+    for (int k = 0; k < 3; ++k) d.velocity[k] = ex.a[k];
+    op_.reset(new sfk::Operator(d));
+    const std::size_t n = n_elements() * dofs_per_cell;
+    for (auto *v : {&sol_old, &sol_new, &sol_tmp}) v->resize(n);
+    sol_new.upload(nodal(ex, false));
+    sol_old.fill(0.);
+    sol_tmp.fill(0.);
+    const double tn = std::sqrt(ex.a[0] * ex.a[0] + ex.a[1] * ex.a[1] + ex.a[2] * ex.a[2]);
+    time_step = 0.2 / inv_jac[0] / tn / degree;
+  }
+  std::size_t n_elements() const { return std::size_t(n_cells[0]) * n_cells[1] * n_cells[2]; }
+
+  // five-stage, fourth-order low-storage RK (evaluation_dg_advect.h:190-214)
+  void do_time_step(double &current_time)
+  {
+    const double a21 = 970286171893. / 4311952581923., a32 = 6584761158862. / 12103376702013.,
+                 a43 = 2251764453980. / 15575788980749., a54 = 26877169314380. / 34165994151039.;
+    const double b1 = 1153189308089. / 22510343858157., b2 = 1772645290293. / 4653164025191.,
+                 b3 = -1672844663538. / 4480602732383., b4 = 2114624349019. / 3568978502595.,
+                 b5 = 5198255086312. / 14908931495163.;
+    const double dt = time_step;
+    do_rk_stage(sol_new, sol_new, sol_old, sol_tmp, current_time, a21 * dt, (b1 - a21) * dt);
+    do_rk_stage(sol_old, sol_tmp, sol_new, sol_tmp, current_time, a32 * dt, (b2 - a32) * dt);
+    do_rk_stage(sol_new, sol_tmp, sol_old, sol_tmp, current_time, a43 * dt, (b3 - a43) * dt);
+    do_rk_stage(sol_old, sol_tmp, sol_tmp, sol_new, current_time, a54 * dt, (b4 - a54) * dt);
+    do_rk_stage(sol_tmp, sol_new, sol_new, sol_tmp, current_time, b5 * dt, 0.);
+    current_time += dt;
+  }
+  void do_matvec() { do_rk_stage<false>(sol_new, sol_new, sol_tmp, sol_tmp, 0., 0., 0.); }
+
+  // max |du/dt_exact - M^{-1}L(u)| at the GLL nodes (:221-251)
+  double verify_derivative()
+  {
+    do_matvec();
+    const std::vector<Number> r = sol_tmp.download();
+    const std::vector<Number> ex = nodal(ExactSolution<dim>(0.), true);
+    double err = 0;
+    for (std::size_t i = 0; i < r.size(); ++i) err = std::max(err, std::abs(double(ex[i]) - double(r[i])));
+    return err;
+  }
+  // max error against the travelling wave at time t (:253-283)
+  double compute_max_error(const double t)
+  {
+    const std::vector<Number> u = sol_new.download();
+    const std::vector<Number> ex = nodal(ExactSolution<dim>(t), false);
+    double err = 0;
+    for (std::size_t i = 0; i < u.size(); ++i) err = std::max(err, std::abs(double(ex[i]) - double(u[i])));
+    return err;
+  }
+
+  template <bool evaluate_stage = true>
+  void do_rk_stage(const sfk::DeviceVector<Number> &src, const sfk::DeviceVector<Number> &update_vec,
+                   sfk::DeviceVector<Number> &vec_np, sfk::DeviceVector<Number> &vec_tm, const double /*time*/,
+                   const Number coefficient_np, const Number coefficient_tm)
+  {
+    if (evaluate_stage) {
+      // the stage input must stay intact: stage through the spare vector
+      // when the reference's call pattern writes the input (stage 5 writes
+      // np over upd, which is allowed; src is never overwritten there)
+      sfk::check(sfk_rk_stage(op_->get(), src.data(), update_vec.data(), vec_np.data(), vec_tm.data(),
+                              coefficient_np, coefficient_tm, nullptr));
+    } else {
+      sfk::check(sfk_apply(op_->get(), src.data(), vec_np.data(), nullptr));
+    }
+    sfk::check(sfk_synchronize(nullptr));
+  }
+
+  unsigned int n_cells[3] = {1, 1, 1};
+  double inv_jac[3] = {1., 1., 1.};
+  double time_step = 0.;
+  sfk::DeviceVector<Number> sol_old, sol_new, sol_tmp;
+
+private:
+  std::vector<Number> nodal(const ExactSolution<dim> &ex, bool derivative) const
+  {
+    std::vector<double> gp(degree + 1), gw(degree + 1), gl(degree + 1);
+    sfk::check(sfk_quadrature(degree + 1, gp.data(), gw.data(), gl.data()));
+    std::vector<Number> v(n_elements() * dofs_per_cell);
+    std::size_t idx = 0;
+    for (unsigned iz = 0; iz < n_cells[2]; ++iz)
+      for (unsigned iy = 0; iy < n_cells[1]; ++iy)
+        for (unsigned ix = 0; ix < n_cells[0]; ++ix)
+          for (int lz = 0; lz <= degree; ++lz)
+            for (int ly = 0; ly <= degree; ++ly)
+              for (int lx = 0; lx <= degree; ++lx, ++idx) {
+                const double p[3] = {(ix + gl[lx]) / inv_jac[0], (iy + gl[ly]) / inv_jac[1], (iz + gl[lz]) / inv_jac[2]};
+                v[idx] = Number(derivative ? ex.time_derivative(p) : ex.value(p));
+              }
+    return v;
+  }
+  std::unique_ptr<sfk::Operator> op_;
+};
+
+// ------------------------------------------------------------------ bodies
+template <int dim, int degree, typename Number>
+double EvaluationCellLaplacian<dim, degree, Number>::do_verification()
+{
+  // A_K applied to u = x_d (linear in each cell): the cell integral of
+  // grad v . grad u equals the boundary integral of v du/dn, so
+  // A_K u - (face term on x_d = 1) + (face term on x_d = 0) = 0.
+  const int n = degree + 1;
+  std::vector<double> gp(n), gw(n), gl(n), S(n * n);
+  sfk::check(sfk_quadrature(n, gp.data(), gw.data(), gl.data()));
+  sfk::check(sfk_tables(degree, basis, S.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
+  std::vector<double> b1(n, 0.), v0(n, 0.), v1(n, 0.);
+  // 1D integrals and end values of the basis through its values at Gauss points
+  for (int i = 0; i < n; ++i)
+    for (int q = 0; q < n; ++q) b1[i] += S[i * n + q] * gw[q];
+  {
+    // end values: interpolate phi_i from the Gauss points (exact for degree p)
+    std::vector<double> l0(n, 1.), l1(n, 1.);
+    for (int q = 0; q < n; ++q)
+      for (int m = 0; m < n; ++m)
+        if (m != q) {
+          l0[q] *= (0. - gp[m]) / (gp[q] - gp[m]);
+          l1[q] *= (1. - gp[m]) / (gp[q] - gp[m]);
+        }
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < n; ++q) {
+        v0[i] += l0[q] * S[i * n + q];
+        v1[i] += l1[q] * S[i * n + q];
+      }
+  }
+  const std::vector<double> ij = sfk::reference_diagonal_jacobian();
+  // nodal values of x_d: for the GLL basis the node positions, for the
+  // Hermite-like basis the coefficients of the linear function x
+  std::vector<double> lin(n);
+  if (basis == SFK_BASIS_GLL) {
+    lin = gl;
+  } else {
+    // x = sum_i c_i phi_i(x): phi_0 + phi_1 carry the value and slope at 0
+    // (phi_1'(0) = -phi_0'(0)), solve on the Gauss points
+    std::vector<double> A(n * n), rhs(n);
+    for (int q = 0; q < n; ++q) {
+      rhs[q] = gp[q];
+      for (int i = 0; i < n; ++i) A[q * n + i] = S[i * n + q];
+    }
+    for (int c = 0; c < n; ++c) // Gaussian elimination with partial pivoting
+    {
+      int piv = c;
+      for (int r = c + 1; r < n; ++r)
+        if (std::abs(A[r * n + c]) > std::abs(A[piv * n + c])) piv = r;
+      for (int k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
+      std::swap(rhs[c], rhs[piv]);
+      for (int r = c + 1; r < n; ++r) {
+        const double f = A[r * n + c] / A[c * n + c];
+        for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+        rhs[r] -= f * rhs[c];
+      }
+    }
+    for (int c = n - 1; c >= 0; --c) {
+      double s = rhs[c];
+      for (int k = c + 1; k < n; ++k) s -= A[c * n + k] * lin[k];
+      lin[c] = s / A[c * n + c];
+    }
+  }
+  double max_error = 0;
+  for (int test = 0; test < 3; ++test) {
+    std::vector<Number> in(n_cells_ * dofs_per_cell);
+    for (unsigned c = 0; c < n_cells_; ++c)
+      for (int k = 0, q = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i, ++q) {
+            const int idx[3] = {i, j, k};
+            in[c * dofs_per_cell + q] = Number(lin[idx[test]] / ij[test]);
+          }
+    input_array.upload(in);
+    matrix_vector_product();
+    const std::vector<Number> out = output_array.download();
+    int other[2], o = 0;
+    for (int e = 0; e < 3; ++e)
+      if (e != test) other[o++] = e;
+    const double fac = 1. / (ij[other[0]] * ij[other[1]]);
+    for (unsigned c = 0; c < n_cells_; ++c)
+      for (int k = 0, q = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i, ++q) {
+            const int idx[3] = {i, j, k};
+            const double ref = fac * (v1[idx[test]] - v0[idx[test]]) * b1[idx[other[0]]] * b1[idx[other[1]]];
+            max_error = std::max(max_error, std::abs(double(out[c * dofs_per_cell + q]) - ref));
+          }
+  }
+  return max_error;
+}
+
+template <int dim, int degree, typename Number>
+void EvaluationDGLaplacian<dim, degree, Number>::emulate_cheby_vector_updates()
+{
+  // tmp = 0.6 tmp + 0.45 diag (new - rhs); new = old - tmp (:556-601)
+  sfk::check(sfk_vec_cheby_update(sfk::precision_of<Number>(), (long long)sol_new.size(), sol_new.data(),
+                                  sol_rhs.data(), sol_old.data(), mat_diagonal.data(), sol_new.data(),
+                                  sol_tmp.data(), 0.45, 0.6, nullptr));
+  sfk::check(sfk_synchronize(nullptr));
+}
