@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="headline matvec only (profiling)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="plain copy kernel of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is a paragraph of DESIGN.md); "
@@ -145,6 +147,15 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.init()
     extra = {}
+    if args.copy_probe:
+        n = 64 * 1024 * 1024  # 512 MiB of FP64 read + 512 MiB written per launch
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 5}))
+        return
 
     # ------------------------------------------------------------ headline
     degree, nc = 5, (32, 32, 32)

@@ -183,7 +183,12 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   for (DevBuf *bb : {&rs, &es, &zs, &ps, &qs, &zero, &tms, &rold, &inv_s})
     SOLVER_HIP(hipMalloc(&bb->p, ns));
   const hipStream_t s = S.s;
-  SOLVER_HIP(vec_fill(f32, zero.p, n, 0., s));
+  // every work vector starts at 0 (a fresh allocation may hold NaN bit
+  // patterns, and 0 * NaN would poison the first Chebyshev step)
+  for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
+    SOLVER_HIP(vec_fill(false, bb->p, n, 0., s));
+  for (DevBuf *bb : {&rs, &es, &zs, &ps, &qs, &zero, &tms, &rold, &inv_s})
+    SOLVER_HIP(vec_fill(f32, bb->p, n, 0., s));
   // point-Jacobi diagonal (FP64), inverse in both precisions
   SOLVER_ST(sfk_compute_diagonal(op, d64.p, stream));
   hipLaunchKernelGGL(inv_diag_kernel, dim3(grid_of(n)), dim3(256), 0, s, (const double *)d64.p, (double *)inv64.p,

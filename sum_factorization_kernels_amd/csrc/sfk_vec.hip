@@ -68,7 +68,7 @@ template <typename T>
 __global__ void axpby_kernel(long long n, T a, const T *x, T b, T *y)
 {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    y[i] = a * x[i] + b * y[i];
+    y[i] = b != T(0) ? a * x[i] + b * y[i] : a * x[i]; // b == 0: y is write-only
 }
 
 template <typename T>
@@ -77,7 +77,8 @@ __global__ void cheby_kernel(long long n, const T *au, const T *rhs, const T *so
 {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const T res = au[i] - rhs[i];
-    const T t = c_tm * tm[i] + c_np * diag[i] * res;
+    // c_tm == 0 starts a recurrence: the previous tm is not read
+    const T t = (c_tm != T(0) ? c_tm * tm[i] : T(0)) + c_np * diag[i] * res;
     tm[i] = t;
     sol_new[i] = sol_old[i] - t;
   }
