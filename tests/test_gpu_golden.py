@@ -1,0 +1,116 @@
+"""GPU kernels (through the C ABI) against the reference's own outputs:
+golden fixtures written from the compiled reference (tests/golden/), with
+the reference-compatible SIPG mode (SFK_FLAG_REF_X_SWEEP) and the cells the
+reference's x-wrap defect affects masked."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import rel_err, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+sfk = pytest.importorskip("sum_factorization_kernels_amd")
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-12
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+@pytest.mark.parametrize("degree", [1, 3, 4, 5, 6, 7, 8])
+def test_cell_laplacian_golden(cuda, degree):
+    g = load(f"cell_laplacian_p{degree}.npz")
+    n3 = (degree + 1) ** 3
+    nc = int(g["n_cells"][0])
+    src = np.random.default_rng(int(g["seed"])).standard_normal(nc * n3)
+    op = sfk.Operator(sfk.CELL_LAPLACE, degree, (nc, 1, 1), basis=sfk.BASIS_HERMITE, inv_jac=g["inv_jac"])
+    assert rel_err(to_np(op.apply(to_dev(src))), g["out"]) < TOL
+
+
+@pytest.mark.parametrize("degree", [1, 3, 4, 5, 6, 7, 8])
+def test_dg_sipg_reference_mode_golden(cuda, degree):
+    g = load(f"dg_laplacian_p{degree}.npz")
+    nc = [int(x) for x in g["n_cells"]]
+    N = (degree + 1) ** 3 * int(np.prod(nc))
+    rng = np.random.default_rng(int(g["seed"]))
+    src, rhs, diag, tm = rng.standard_normal(N), rng.standard_normal(N), rng.uniform(0.5, 1.5, N), \
+        rng.standard_normal(N)
+    mask = np.repeat(g["mask"].astype(bool), (degree + 1) ** 3)
+    op = sfk.Operator(sfk.DG_SIPG, degree, nc, inv_jac=g["inv_jac"], flags=sfk.FLAG_REF_X_SWEEP)
+    out = to_np(op.apply(to_dev(src)))
+    assert rel_err(out[mask], g["matvec"][mask]) < TOL
+    d_new, d_tm = to_dev(np.zeros(N)), to_dev(tm)
+    op.cheby_step(to_dev(rhs), to_dev(src), to_dev(diag), d_new, d_tm, float(g["c_np"]), float(g["c_tm"]))
+    assert rel_err(to_np(d_new)[mask], g["cheby_new"][mask]) < TOL
+    assert rel_err(to_np(d_tm)[mask], g["cheby_tm"][mask]) < TOL
+
+
+@pytest.mark.parametrize("degree", range(1, 9))
+def test_dg_advect_golden(cuda, degree):
+    g = load(f"dg_advect_p{degree}.npz")
+    nc = [int(x) for x in g["n_cells"]]
+    N = (degree + 1) ** 3 * int(np.prod(nc))
+    rng = np.random.default_rng(int(g["seed"]))
+    src, upd, tm = rng.standard_normal(N), rng.standard_normal(N), rng.standard_normal(N)
+    op = sfk.Operator(sfk.DG_ADVECT, degree, nc, inv_jac=g["inv_jac"], velocity=g["velocity"])
+    assert rel_err(to_np(op.apply(to_dev(src))), g["matvec"]) < TOL
+    d_np, d_tm = to_dev(np.zeros(N)), to_dev(tm)
+    op.rk_stage(to_dev(src), to_dev(upd), d_np, d_tm, float(g["c_np"]), float(g["c_tm"]))
+    assert rel_err(to_np(d_np), g["stage_np"]) < TOL
+    assert rel_err(to_np(d_tm), g["stage_tm"]) < TOL
+
+
+@pytest.mark.parametrize("degree", range(1, 8))
+def test_advect_derivative_kat_gpu(cuda, degree):
+    """the reference's verify_derivative value (evaluation_dg_advect.h:221-251)"""
+    from test_oracle_golden import exact_solution, gll_nodal
+    k = load("advect_kat.npz")
+    nc = [int(x) for x in k["n_cells"]]
+    ij = k[f"inv_jac_p{degree}"]
+    u = gll_nodal(degree, nc, ij, lambda x, y, z: exact_solution(x, y, z, 0.0))
+    dudt = gll_nodal(degree, nc, ij, lambda x, y, z: exact_solution(x, y, z, 0.0, deriv=True))
+    op = sfk.Operator(sfk.DG_ADVECT, degree, nc, inv_jac=ij, velocity=(1.0, 0.15, -0.05))
+    err = np.abs(dudt - to_np(op.apply(to_dev(u)))).max()
+    assert abs(err - k[f"p{degree}"][0]) <= 1e-10 * max(1.0, err)
+
+
+def lsrk_time_loop(op, u0, n_steps, dt):
+    """the five-stage low-storage RK loop of evaluation_dg_advect.h:190-214
+    with the reference's vector roles (sol_new, sol_old, sol_tmp)"""
+    import torch
+    a21, a32 = 970286171893. / 4311952581923., 6584761158862. / 12103376702013.
+    a43, a54 = 2251764453980. / 15575788980749., 26877169314380. / 34165994151039.
+    b1, b2 = 1153189308089. / 22510343858157., 1772645290293. / 4653164025191.
+    b3, b4 = -1672844663538. / 4480602732383., 2114624349019. / 3568978502595.
+    b5 = 5198255086312. / 14908931495163.
+    new, old, tmp = u0.clone(), torch.zeros_like(u0), torch.zeros_like(u0)
+    t = 0.0
+    for _ in range(n_steps):
+        op.rk_stage(new, new, old, tmp, a21 * dt, (b1 - a21) * dt)
+        op.rk_stage(old, tmp, new, tmp, a32 * dt, (b2 - a32) * dt)
+        op.rk_stage(new, tmp, old, tmp, a43 * dt, (b3 - a43) * dt)
+        op.rk_stage(old, tmp, tmp, new, a54 * dt, (b4 - a54) * dt)
+        op.rk_stage(tmp, new, new, tmp, b5 * dt, 0.0)
+        t += dt
+    return new, t
+
+
+@pytest.mark.parametrize("degree", range(1, 8))
+def test_advect_time_loop_kat_gpu(cuda, degree):
+    """error against the travelling wave after 20 LSRK steps equals the
+    reference's compute_max_error value (evaluation_dg_advect.h:253-283)"""
+    from test_oracle_golden import exact_solution, gll_nodal
+    k = load("advect_kat.npz")
+    nc = [int(x) for x in k["n_cells"]]
+    ij = k[f"inv_jac_p{degree}"]
+    _, err_sol, t_end, n_steps = k[f"p{degree}"]
+    u0 = gll_nodal(degree, nc, ij, lambda x, y, z: exact_solution(x, y, z, 0.0))
+    op = sfk.Operator(sfk.DG_ADVECT, degree, nc, inv_jac=ij, velocity=(1.0, 0.15, -0.05))
+    dt = 0.2 / ij[0] / np.sqrt(1.0 + 0.15 ** 2 + 0.05 ** 2) / degree  # evaluation_dg_advect.h:160-165
+    u, t = lsrk_time_loop(op, to_dev(u0), int(n_steps), dt)
+    assert abs(t - t_end) < 1e-12
+    ex = gll_nodal(degree, nc, ij, lambda x, y, z: exact_solution(x, y, z, t))
+    err = np.abs(ex - to_np(u)).max()
+    assert abs(err - err_sol) <= 1e-9 * max(1.0, err_sol)
