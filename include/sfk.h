@@ -149,6 +149,39 @@ sfk_status sfk_grad_coefficient(sfk_op *op, const void *u, const void *lambda, v
 /* b_i = integral of f phi_i (continuous Laplacian; Dirichlet rows 0) */
 sfk_status sfk_op_rhs_constant(sfk_op *op, double f, void *b, void *stream);
 
+/* ---- workgroup schedule and autotuner ----------------------------------
+ * Replaces the hand-chosen loop blocking of the reference drivers
+ * (evaluator.blx/bly/blz, legacy/src/test_dg_integrate.cc:64-66,
+ * legacy/src/test_dg_advect.cc): on the GPU the schedule of an operator
+ * kernel is the number of cells per workgroup, the order in which
+ * workgroups walk the mesh and, for the continuous Laplacian, the scatter
+ * strategy.  Results do not depend on the schedule (bit-identical for DG
+ * and the colored continuous scatter; the atomic scatter sums in a
+ * run-dependent order). */
+typedef struct sfk_schedule {
+  int cells_per_block; /* cells per workgroup; 0 = the degree's default */
+  int block_order;     /* 0 linear, 1 one contiguous range of cells per XCD */
+  int cont_scatter;    /* continuous Laplacian: -1 automatic, 0 zero fill +
+                          atomic adds, 1 colored stores (8 launches) */
+} sfk_schedule;
+
+/* Current schedule, with defaults resolved (cells_per_block > 0). */
+sfk_status sfk_op_get_schedule(const sfk_op *op, sfk_schedule *s);
+/* Sets the schedule; SFK_ERR_INVALID_ARG if cells_per_block exceeds
+ * sfk_op_max_cells_per_block. */
+sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s);
+/* Largest cells_per_block of the operator (workgroup of at most 256 threads,
+ * LDS within the device's per-workgroup limit). */
+sfk_status sfk_op_max_cells_per_block(const sfk_op *op, int *max_cells);
+/* Times sfk_apply for every candidate schedule (`repeats` applies each,
+ * after a warm-up apply, on internal scratch vectors; stream-ordered on
+ * `stream`), sets the fastest on the handle and returns it in *best with its
+ * time per apply in *best_seconds (either may be NULL).  Candidates: every
+ * cells_per_block in [1, max] x both block orders x, for the continuous
+ * Laplacian, both scatters.  Requires the operator's data (metric,
+ * coefficient, velocity) to be set. */
+sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_seconds, void *stream);
+
 /* ---- solver (continuous Laplacian) ------------------------------------
  * Preconditioned CG in FP64 with a Chebyshev(degree) point-Jacobi
  * preconditioner whose bounds come from a Lanczos estimate of

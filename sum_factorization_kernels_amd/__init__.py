@@ -48,6 +48,11 @@ class SolveStats(ctypes.Structure):
                 ("matvecs_outer", ctypes.c_int), ("matvecs_inner", ctypes.c_int)]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("cells_per_block", ctypes.c_int), ("block_order", ctypes.c_int),
+                ("cont_scatter", ctypes.c_int)]
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -91,6 +96,10 @@ def lib():
         L.sfk_layout_to_interleaved.argtypes = L.sfk_layout_to_canonical.argtypes
         L.sfk_vec_fill.argtypes = [ctypes.c_int, _vp, _ll, ctypes.c_double, _vp]
         L.sfk_vec_dot.argtypes = [ctypes.c_int, _vp, _vp, _ll, _dp, _vp]
+        L.sfk_op_get_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
+        L.sfk_op_set_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
+        L.sfk_op_max_cells_per_block.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+        L.sfk_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Schedule), _dp, _vp]
         _lib = L
     return _lib
 
@@ -247,6 +256,30 @@ class Operator:
         gf = torch.zeros(3 * self.n_face_q, dtype=self.dtype, device="cuda")
         check(lib().sfk_grad_velocity(self.handle, _dptr(u), _dptr(lam), _dptr(gc), _dptr(gf), _stream(stream)))
         return gc, gf
+
+    @property
+    def schedule(self):
+        """(cells_per_block, block_order, cont_scatter) of the operator kernel"""
+        sc = Schedule()
+        check(lib().sfk_op_get_schedule(self.handle, ctypes.byref(sc)))
+        return (sc.cells_per_block, sc.block_order, sc.cont_scatter)
+
+    def set_schedule(self, cells_per_block=0, block_order=1, cont_scatter=-1):
+        sc = Schedule(int(cells_per_block), int(block_order), int(cont_scatter))
+        check(lib().sfk_op_set_schedule(self.handle, ctypes.byref(sc)))
+
+    @property
+    def max_cells_per_block(self):
+        m = ctypes.c_int()
+        check(lib().sfk_op_max_cells_per_block(self.handle, ctypes.byref(m)))
+        return m.value
+
+    def tune(self, repeats=10, stream=None):
+        """Time every candidate schedule, keep the fastest; returns (schedule, seconds per apply)."""
+        sc = Schedule()
+        t = ctypes.c_double()
+        check(lib().sfk_tune(self.handle, int(repeats), ctypes.byref(sc), ctypes.byref(t), _stream(stream)))
+        return (sc.cells_per_block, sc.block_order, sc.cont_scatter), t.value
 
     def grad_coefficient(self, u, lam, stream=None):
         import torch

@@ -23,11 +23,7 @@ constexpr int adv_cells_per_block()
 {
   return 256 / (N * N) > 0 ? 256 / (N * N) : 1;
 }
-template <int N>
-constexpr int adv_threads()
-{
-  return ((adv_cells_per_block<N>() * N * N + 63) / 64) * 64;
-}
+constexpr int kAdvMaxThreads = 256;
 
 template <int N, typename T>
 struct AdvArgs {
@@ -42,6 +38,8 @@ struct AdvArgs {
   T jxwf[3];
   long long ncells;
   int nx, ny, nz;
+  int cb;    // cells per workgroup
+  int order; // block order (logical_block)
 };
 
 // normal velocity a.n (outward from the cell) at face point fa of face (d, s)
@@ -155,17 +153,18 @@ __device__ __forceinline__ void adv_dir(const AdvArgs<N, T> &a, const T *Acell, 
 }
 
 template <int N, typename T, int KIND, bool VAR, bool STAGE>
-__global__ void __launch_bounds__(adv_threads<N>()) advect_kernel(const AdvArgs<N, T> a)
+__global__ void __launch_bounds__(kAdvMaxThreads) advect_kernel(const AdvArgs<N, T> a)
 {
-  constexpr int CB = adv_cells_per_block<N>();
   constexpr int N2 = N * N, N3 = N * N * N;
   constexpr bool ADJ = KIND == 1;
-  __shared__ T sA[CB * N3];
-  __shared__ T sB[CB * N3];
-  __shared__ T sF[KIND == 2 ? 1 : CB * 6 * N2];
+  const int CB = a.cb;
+  // dynamic LDS: A [CB][N3], B [CB][N3], F [CB][6][N2] (not for M^{-1})
+  T *sA = dyn_lds<T>();
+  T *sB = sA + CB * N3;
+  T *sF = sB + CB * N3;
   const Tab<N, T> &tb = a.tab;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
-  const long long cell0 = (long long)blockIdx.x * CB, cell = cell0 + c;
+  const long long cell0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB, cell = cell0 + c;
   const bool act = c < CB && cell < a.ncells;
   T *A = sA + (act ? c : 0) * N3;
   T *B = sB + (act ? c : 0) * N3;
@@ -430,27 +429,27 @@ static hipError_t launch_advect_n(const AdvectLaunch &L, hipStream_t s)
   a.nx = L.nc[0];
   a.ny = L.nc[1];
   a.nz = L.nc[2];
-  constexpr int CB = adv_cells_per_block<N>();
-  const dim3 grid((unsigned)((L.ncells + CB - 1) / CB)), block(adv_threads<N>());
+  const int CB = L.cb > 0 ? L.cb : adv_cells_per_block<N>();
+  if (CB * N * N > kAdvMaxThreads) return hipErrorInvalidConfiguration;
+  a.cb = CB;
+  a.order = L.order;
+  const size_t lds = size_t(CB) * (2 * N * N * N + (L.kind == 2 ? 0 : 6 * N * N)) * sizeof(T);
+  const dim3 grid((unsigned)((L.ncells + CB - 1) / CB)), block(((CB * N * N + 63) / 64) * 64);
   const bool var = L.cell_vel != nullptr;
+  decltype(&advect_kernel<N, T, 0, false, false>) kern = nullptr;
   switch (L.kind) {
   case 0:
-    if (var) {
-      if (L.stage) hipLaunchKernelGGL((advect_kernel<N, T, 0, true, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((advect_kernel<N, T, 0, true, false>), grid, block, 0, s, a);
-    } else {
-      if (L.stage) hipLaunchKernelGGL((advect_kernel<N, T, 0, false, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((advect_kernel<N, T, 0, false, false>), grid, block, 0, s, a);
-    }
+    if (var) kern = L.stage ? advect_kernel<N, T, 0, true, true> : advect_kernel<N, T, 0, true, false>;
+    else kern = L.stage ? advect_kernel<N, T, 0, false, true> : advect_kernel<N, T, 0, false, false>;
     break;
-  case 1:
-    if (var) hipLaunchKernelGGL((advect_kernel<N, T, 1, true, false>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((advect_kernel<N, T, 1, false, false>), grid, block, 0, s, a);
-    break;
-  case 2: hipLaunchKernelGGL((advect_kernel<N, T, 2, false, false>), grid, block, 0, s, a); break;
-  case 3: hipLaunchKernelGGL((advect_kernel<N, T, 3, true, false>), grid, block, 0, s, a); break;
+  case 1: kern = var ? advect_kernel<N, T, 1, true, false> : advect_kernel<N, T, 1, false, false>; break;
+  case 2: kern = advect_kernel<N, T, 2, false, false>; break;
+  case 3: kern = advect_kernel<N, T, 3, true, false>; break;
   default: return hipErrorInvalidValue;
   }
+  const hipError_t e = allow_dynamic_lds((const void *)kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, grid, block, lds, s, a);
   return hipGetLastError();
 }
 

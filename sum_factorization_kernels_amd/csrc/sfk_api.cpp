@@ -92,6 +92,9 @@ void fill_cell_launch(const sfk_op *op, CellLaunch &L)
   L.nrhs = op->d.n_rhs;
   L.vstride = op->nvec;
   L.coef = op->coef;
+  L.cont_scatter = op->cont_scatter;
+  L.cb = op->sched_cb;
+  L.order = op->sched_order;
 }
 
 } // namespace sfk
@@ -253,7 +256,6 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream)
     L.mode = d.kind == SFK_CELL_LAPLACE ? 0 : 1;
     L.src = src;
     L.dst = dst;
-    if (L.mode == 1) SFK_CHECK_HIP(vec_fill(fp32, dst, op->nvec * d.n_rhs, 0., s));
     SFK_CHECK_HIP(launch_cell(L, s));
     return SFK_OK;
   }

@@ -39,6 +39,7 @@ constexpr int threads_per_block()
 {
   return ((cells_per_block<N>() * N * N + 63) / 64) * 64;
 }
+constexpr int kCellMaxThreads = 256;
 
 template <int N, typename T>
 struct CellArgs {
@@ -58,20 +59,13 @@ struct CellArgs {
   T tau[3];        // penalty (p+1)^2 / h_d
   int nrhs;
   long long vstride; // elements between right-hand sides
+  // continuous: colour of this launch and the row/segment decomposition
+  // (a workgroup = CB consecutive cells of one x row; colour = parity of
+  // (row y, row z, segment) so that a launch writes every node at most once)
+  int color, nseg, nss, nry, nrz;
+  int cb;    // cells per workgroup (continuous colored: cells per segment)
+  int order; // block order (logical_block)
 };
-
-template <int N, typename T, Mode MODE>
-__device__ __forceinline__ long long node_index(const CellArgs<N, T> &a, long long cell, int i, int j,
-                                                int k, bool &boundary)
-{
-  constexpr int p = N - 1;
-  const int cx = (int)(cell % a.nx);
-  const long long r = cell / a.nx;
-  const int cy = (int)(r % a.ny), cz = (int)(r / a.ny);
-  const int X = cx * p + i, Y = cy * p + j, Z = cz * p + k;
-  boundary = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
-  return ((long long)Z * a.gy + Y) * a.gx + X;
-}
 
 // face terms of SIPG along one line (Cartesian): the thread owns line l of
 // direction DIR, q[] are the values at the Gauss points of the line; adds
@@ -129,22 +123,87 @@ __device__ __forceinline__ void grad_phase(const CellArgs<N, T> &a, const T *Ace
   if constexpr (MODE == Mode::DG) sipg_faces<N, DIR, T>(a, q, F, l, r);
 }
 
-template <int N, typename T, Mode MODE, bool CHEBY, bool REFX, bool COEF>
-__global__ void __launch_bounds__(threads_per_block<N>()) cell_kernel(const CellArgs<N, T> a)
+// Colored store of one x-line of a continuous cell.  A workgroup owns CB
+// consecutive cells of one x row (a "segment"); the units (row y, row z,
+// segment) are coloured by the parity of each index, so two units of one
+// colour share no node and a launch writes every node at most once.  The
+// colours are launched in ascending order; for every node the unit with the
+// smallest colour among those that touch it stores (Dirichlet nodes: 0) and
+// later ones add, which makes a zero fill and atomics unnecessary.  Inside a
+// segment the first x-plane of cell c+1 (left in LDS by its threads) is
+// added to the last x-plane of cell c.
+template <int N, typename T>
+__device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&o)[N], const T *sB, int c, int l,
+                                                   int cx, int cy, int cz, int seg, int cx0, int cx_end, T *dst)
 {
-  constexpr int CB = cells_per_block<N>();
+  constexpr int p = N - 1, N3 = N * N * N;
+  const int j = l % N, k = l / N;
+  if (cx + 1 < cx_end) o[p] += sB[(c + 1) * N3 + l];
+  const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
+  const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;
+  // smallest colour of the units covering rows [ylo,yhi] x [zlo,zhi] x
+  // segments [slo,shi] (each range has one or two entries)
+  auto min_color = [&](int slo, int shi) {
+    const int py = ylo == yhi ? (ylo & 1) : 0, pz = zlo == zhi ? (zlo & 1) : 0, ps = slo == shi ? (slo & 1) : 0;
+    return py + 2 * pz + 4 * ps;
+  };
+  const int m_mid = min_color(seg, seg);
+  const int Y = cy * p + j, Z = cz * p + k;
+  const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
+  const long long g0 = ((long long)Z * a.gy + Y) * a.gx + (long long)cx * p;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i == 0 && cx > cx0) continue; // added into the previous cell's plane
+    int m = m_mid;
+    if (i == 0 && seg > 0) m = min_color(seg - 1, seg);
+    if (i == p && cx + 1 == cx_end && seg + 1 < a.nseg) m = min_color(seg, seg + 1);
+    const int X = cx * p + i;
+    const bool bd = byz || X == 0 || X == a.gx - 1;
+    if (m == a.color) dst[g0 + i] = bd ? T(0) : o[i];
+    else if (!bd) dst[g0 + i] += o[i];
+  }
+}
+
+template <int N, typename T, Mode MODE, bool CHEBY, bool REFX, bool COEF>
+__global__ void __launch_bounds__(kCellMaxThreads) cell_kernel(const CellArgs<N, T> a)
+{
   constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
   constexpr bool FACES = MODE == Mode::DG;
-  __shared__ T sA[CB * N3];
-  __shared__ T sB[CB * N3];
-  __shared__ T sF[FACES ? CB * 12 * N2 : 1];
+  const int CB = a.cb;
+  // dynamic LDS: A [CB][N3], B [CB][N3], F [CB][12][N2] (DG only)
+  T *sA = dyn_lds<T>();
+  T *sB = sA + CB * N3;
+  T *sF = sB + CB * N3;
 
   const Tab<N, T> &tb = a.tab;
   const int t = threadIdx.x;
   const int c = t / N2, l = t % N2;
-  const long long cell0 = (long long)blockIdx.x * CB;
+  const unsigned blk = logical_block(blockIdx.x, gridDim.x, a.order);
+  long long cell0 = (long long)blk * CB;
+  // continuous: coordinates of this thread's cell; a colour >= 0 selects the
+  // row-segment decomposition, a negative one the cell-linear atomic variant
+  int seg = 0, ccx = 0, ccy = 0, ccz = 0, cx0 = 0, cx_end = 0;
+  bool act = c < CB && cell0 + c < a.ncells;
+  if constexpr (MODE == Mode::Cont) {
+    if (a.color >= 0) {
+      const int si = (int)(blk % a.nss);
+      const int rest = (int)(blk / a.nss);
+      seg = (a.color >> 2 & 1) + 2 * si;
+      ccy = (a.color & 1) + 2 * (rest % a.nry);
+      ccz = (a.color >> 1 & 1) + 2 * (rest / a.nry);
+      cx0 = seg * CB;
+      cx_end = cx0 + CB < a.nx ? cx0 + CB : a.nx;
+      ccx = cx0 + c;
+      cell0 = ((long long)ccz * a.ny + ccy) * a.nx + cx0;
+      act = c < CB && ccx < cx_end;
+    } else {
+      const int ci = (int)(cell0 + c);
+      ccx = ci % a.nx;
+      ccy = (ci / a.nx) % a.ny;
+      ccz = ci / a.nx / a.ny;
+    }
+  }
   const long long cell = cell0 + c;
-  const bool act = c < CB && cell < a.ncells;
   T *A = sA + (act ? c : 0) * N3;
   T *B = sB + (act ? c : 0) * N3;
   T *F = sF + (FACES && act ? c * 12 * N2 : 0);
@@ -156,11 +215,14 @@ __global__ void __launch_bounds__(threads_per_block<N>()) cell_kernel(const Cell
     if (act) {
       T u[N], v[N];
       if constexpr (MODE == Mode::Cont) {
+        const int X = ccx * p + l % N, Y = ccy * p + l / N;
+        const bool bxy = X == 0 || Y == 0 || X == a.gx - 1 || Y == a.gy - 1;
+        const long long g0 = (long long)ccz * p * a.gy * a.gx + (long long)Y * a.gx + X;
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-          bool bd;
-          const long long g = node_index<N, T, MODE>(a, cell, l % N, l / N, k, bd);
-          u[k] = bd ? T(0) : src[g];
+          const int Z = ccz * p + k;
+          const bool bd = bxy || Z == 0 || Z == a.gz - 1;
+          u[k] = bd ? T(0) : src[g0 + (long long)k * a.gy * a.gx];
         }
       } else {
 #pragma unroll
@@ -270,20 +332,36 @@ __global__ void __launch_bounds__(threads_per_block<N>()) cell_kernel(const Cell
     }
     __syncthreads();
     // --------------------------------------------------------------- Bx
+    T o[N];
     if (act) {
-      T u[N], o[N];
+      T u[N];
       lds_read_line<N, N, 0>(A, l, u);
       if constexpr (REFX)
         eo_apply<N, 1>(tb.Sf, u, o); // evaluation_dg_laplacian.h:481-483
       else
         eo_apply<N, 1>(tb.Sb, u, o);
       if constexpr (MODE == Mode::Cont) {
+        // x-merge: the first x-plane of cell c+1 is the last one of cell c
+        if (a.color >= 0) B[l] = o[0];
+      }
+    }
+    if constexpr (MODE == Mode::Cont) {
+      if (a.color >= 0) {
+        __syncthreads();
+        if (act) cont_store_colored<N, T>(a, o, sB, c, l, ccx, ccy, ccz, seg, cx0, cx_end, dst);
+      } else if (act) {
+        const int Y = ccy * p + l % N, Z = ccz * p + l / N;
+        const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
+        const long long g0 = ((long long)Z * a.gy + Y) * a.gx + (long long)ccx * p;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          bool bd;
-          const long long g = node_index<N, T, MODE>(a, cell, i, l % N, l / N, bd);
-          if (!bd) atomicAdd(dst + g, o[i]);
+          const int X = ccx * p + i;
+          if (!(byz || X == 0 || X == a.gx - 1)) atomicAdd(dst + g0 + i, o[i]);
         }
+      }
+    }
+    if (act) {
+      if constexpr (MODE == Mode::Cont) {
       } else if constexpr (CHEBY) {
         // r = A u - rhs; tm = c_tm*tm + c_np*diag*r; dst = u - tm
         // (evaluation_dg_laplacian.h:485-502)
@@ -546,6 +624,15 @@ static Tab<N, T> make_tab(const Tables1D &t)
   return tb;
 }
 
+// continuous scatter schedule: colored (no atomics, no zero fill) unless the
+// caller asks for the atomic variant or, by default, each of the 8 colour
+// launches would hold fewer workgroups than the chip has CUs
+static bool cont_colored(const CellLaunch &L, int cb)
+{
+  if (L.cont_scatter >= 0) return L.cont_scatter == 1;
+  return L.ncells / cb >= 8 * 256;
+}
+
 template <int N, typename T>
 static hipError_t launch_cell_n(const CellLaunch &L, hipStream_t stream)
 {
@@ -575,33 +662,52 @@ static hipError_t launch_cell_n(const CellLaunch &L, hipStream_t stream)
   }
   a.nrhs = L.nrhs;
   a.vstride = L.vstride;
-  constexpr int CB = cells_per_block<N>();
+  const int CB = L.cb > 0 ? L.cb : cells_per_block<N>();
+  if (CB * N * N > kCellMaxThreads) return hipErrorInvalidConfiguration;
+  a.cb = CB;
+  a.order = L.order;
+  constexpr int N3 = N * N * N;
+  const size_t lds = size_t(CB) * (2 * N3 + (L.mode == 2 ? 12 * N * N : 0)) * sizeof(T);
   const unsigned blocks = (unsigned)((L.ncells + CB - 1) / CB);
-  const dim3 grid(blocks), block(threads_per_block<N>());
+  const dim3 grid(blocks), block(((CB * N * N + 63) / 64) * 64);
+  auto launch = [&](auto kern, dim3 g) {
+    const hipError_t e = allow_dynamic_lds((const void *)kern, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, g, block, lds, stream, a);
+    return hipGetLastError();
+  };
   if (L.mode == 0) {
-    if (L.coef)
-      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Local, false, false, true>), grid, block, 0, stream, a);
-    else
-      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Local, false, false, false>), grid, block, 0, stream, a);
+    return launch(L.coef ? cell_kernel<N, T, Mode::Local, false, false, true>
+                         : cell_kernel<N, T, Mode::Local, false, false, false>, grid);
   } else if (L.mode == 1) {
-    if (L.coef)
-      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Cont, false, false, true>), grid, block, 0, stream, a);
-    else
-      hipLaunchKernelGGL((cell_kernel<N, T, Mode::Cont, false, false, false>), grid, block, 0, stream, a);
-  } else {
-    if (L.cheby) {
-      if (L.refx)
-        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, true, true, false>), grid, block, 0, stream, a);
-      else
-        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, true, false, false>), grid, block, 0, stream, a);
-    } else {
-      if (L.refx)
-        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, false, true, false>), grid, block, 0, stream, a);
-      else
-        hipLaunchKernelGGL((cell_kernel<N, T, Mode::DG, false, false, false>), grid, block, 0, stream, a);
+    auto kern = L.coef ? cell_kernel<N, T, Mode::Cont, false, false, true>
+                       : cell_kernel<N, T, Mode::Cont, false, false, false>;
+    if (!cont_colored(L, CB)) {
+      a.color = -1;
+      const hipError_t e = vec_fill(std::is_same<T, float>::value, L.dst, L.vstride * L.nrhs, 0., stream);
+      if (e != hipSuccess) return e;
+      return launch(kern, grid);
     }
+    a.nseg = (a.nx + CB - 1) / CB;
+    // colours in ascending order: the first unit to touch a node stores it
+    for (int color = 0; color < 8; ++color) {
+      const int ps = color >> 2 & 1, py = color & 1, pz = color >> 1 & 1;
+      a.color = color;
+      a.nss = (a.nseg - ps + 1) / 2;
+      a.nry = (a.ny - py + 1) / 2;
+      a.nrz = (a.nz - pz + 1) / 2;
+      const long long nb = (long long)a.nss * a.nry * a.nrz;
+      if (nb == 0) continue;
+      const hipError_t e = launch(kern, dim3((unsigned)nb));
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
   }
-  return hipGetLastError();
+  if (L.cheby)
+    return launch(L.refx ? cell_kernel<N, T, Mode::DG, true, true, false> : cell_kernel<N, T, Mode::DG, true, false, false>,
+                  grid);
+  return launch(L.refx ? cell_kernel<N, T, Mode::DG, false, true, false> : cell_kernel<N, T, Mode::DG, false, false, false>,
+                grid);
 }
 
 template <int N, typename T>

@@ -142,16 +142,49 @@ __device__ __forceinline__ int face_dof(int layer, int fa)
   else return layer * N * N + fa;
 }
 
+// periodic neighbour of cell c (x fastest); cell counts fit in 32 bits
 __device__ __forceinline__ long long neighbor_cell(long long c, int nx, int ny, int nz, int dir, int side)
 {
-  int ix = (int)(c % nx);
-  long long r = c / nx;
-  int iy = (int)(r % ny);
-  int iz = (int)(r / ny);
+  const int ci = (int)c;
+  int ix = ci % nx;
+  const int r = ci / nx;
+  int iy = r % ny;
+  int iz = r / ny;
   if (dir == 0) ix = side ? (ix + 1 == nx ? 0 : ix + 1) : (ix == 0 ? nx - 1 : ix - 1);
   else if (dir == 1) iy = side ? (iy + 1 == ny ? 0 : iy + 1) : (iy == 0 ? ny - 1 : iy - 1);
   else iz = side ? (iz + 1 == nz ? 0 : iz + 1) : (iz == 0 ? nz - 1 : iz - 1);
   return ((long long)iz * ny + iy) * nx + ix;
+}
+
+// Workgroups of a launch are dispatched round-robin over the chip's 8 XCDs
+// (workgroup b starts on XCD b % 8), and every XCD has its own L2.  Block
+// order 1 hands XCD x the contiguous range [start(x), start(x+1)) of
+// logical blocks, so neighbouring cells, whose face layers and face metric
+// each block re-reads, are processed by one XCD and meet in its L2.  Any
+// order is a bijection of [0, nb): results do not depend on it.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned logical_block(unsigned b, unsigned nb, int order)
+{
+  if (order == 0) return b;
+  const unsigned q = nb / kXcds, r = nb % kXcds, x = b % kXcds, k = b / kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// read-once streams (stored metric): non-temporal loads keep them from
+// evicting the re-read neighbour data out of L2
+template <typename T>
+__device__ __forceinline__ T load_stream(const T *p)
+{
+  return __builtin_nontemporal_load(p);
+}
+
+// dynamic LDS of a kernel (sized by the launch: cells per block is a
+// run-time schedule parameter)
+template <typename T>
+__device__ __forceinline__ T *dyn_lds()
+{
+  extern __shared__ __attribute__((aligned(16))) unsigned char sfk_dyn_lds[];
+  return reinterpret_cast<T *>(sfk_dyn_lds);
 }
 
 } // namespace sfk

@@ -8,6 +8,18 @@
 
 namespace sfk {
 
+// Workgroup schedule fields of the operator launches: cb = cells per
+// workgroup (0: the degree's default), order = block order (0 linear, 1 one
+// contiguous range of cells per XCD; logical_block in sfk_device.h).
+
+// kernels whose dynamic LDS exceeds the default 64 KiB need the limit
+// raised (gfx950 has 160 KiB of LDS per CU)
+inline hipError_t allow_dynamic_lds(const void *kern, size_t bytes)
+{
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 struct CellLaunch {
   const Tables1D *tables = nullptr;
   bool fp32 = false;
@@ -26,6 +38,8 @@ struct CellLaunch {
   long long vstride = 0;
   bool load = false;       // launch_cell_diagonal: load vector of a constant instead
   double load_value = 0.;
+  int cont_scatter = -1;   // continuous: -1 automatic, 0 zero fill + atomics, 1 colored
+  int cb = 0, order = 1;   // schedule
 };
 
 hipError_t launch_cell(const CellLaunch &L, hipStream_t stream);
@@ -45,6 +59,7 @@ struct SipgLaunch {
   const void *cell_metric = nullptr, *face_metric = nullptr;
   long long ncells = 0;
   int nc[3] = {1, 1, 1};
+  int cb = 0, order = 1; // schedule
 };
 hipError_t launch_sipg_general(const SipgLaunch &L, hipStream_t stream);
 
@@ -62,6 +77,7 @@ struct AdvectLaunch {
   long long ncells = 0;
   int nc[3] = {1, 1, 1};
   double inv_jac[3] = {1., 1., 1.};
+  int cb = 0, order = 1; // schedule
 };
 hipError_t launch_advect(const AdvectLaunch &L, hipStream_t stream);
 

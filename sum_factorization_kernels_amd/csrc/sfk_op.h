@@ -18,6 +18,9 @@ struct sfk_op {
   void *cell_vel = nullptr, *face_vel = nullptr;
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // workgroup schedule (sfk_schedule): cells per block (0 = default), block
+  // order, continuous scatter (-1 automatic, 0 atomic, 1 colored)
+  int sched_cb = 0, sched_order = 1, cont_scatter = -1;
 };
 
 namespace sfk {
@@ -30,6 +33,8 @@ inline void fill_sipg_launch(const sfk_op *op, SipgLaunch &L)
   for (int d = 0; d < 3; ++d) L.nc[d] = op->d.n_cells[d];
   L.cell_metric = op->cell_metric;
   L.face_metric = op->face_metric;
+  L.cb = op->sched_cb;
+  L.order = op->sched_order;
 }
 
 inline void fill_advect_launch(const sfk_op *op, AdvectLaunch &L)
@@ -46,6 +51,8 @@ inline void fill_advect_launch(const sfk_op *op, AdvectLaunch &L)
     L.cell_vel = op->cell_vel;
     L.face_vel = op->face_vel;
   }
+  L.cb = op->sched_cb;
+  L.order = op->sched_order;
 }
 
 size_t elem_size(const sfk_op *op);

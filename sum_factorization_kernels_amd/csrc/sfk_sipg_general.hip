@@ -38,14 +38,10 @@ namespace sfk {
 template <int N>
 constexpr int gen_cells_per_block()
 {
-  // 4 N^3 + 12 N^2 values of LDS per cell
+  // default: 4 N^3 + 12 N^2 values of LDS per cell
   return N <= 3 ? 256 / (N * N) : (N == 4 ? 12 : (N == 5 ? 8 : (N == 6 ? 5 : (N == 7 ? 3 : (N == 8 ? 2 : 1)))));
 }
-template <int N>
-constexpr int gen_threads()
-{
-  return ((gen_cells_per_block<N>() * N * N + 63) / 64) * 64;
-}
+constexpr int kGenMaxThreads = 256;
 
 template <int N, typename T>
 struct GenArgs {
@@ -61,6 +57,8 @@ struct GenArgs {
   long long ncells;
   long long nitems; // cells (matvec) or cells * N^3 (diagonal)
   int nx, ny, nz;
+  int cb;    // cells (items) per workgroup
+  int order; // block order (logical_block)
 };
 
 template <int N, int DIR>
@@ -142,16 +140,17 @@ __device__ __forceinline__ void inject_grad(const Tab<N, T> &tb, T *Tx, T *Ty, T
 }
 
 template <int N, typename T, bool CHEBY, bool DIAG>
-__global__ void __launch_bounds__(gen_threads<N>()) sipg_general_kernel(const GenArgs<N, T> a)
+__global__ void __launch_bounds__(kGenMaxThreads) sipg_general_kernel(const GenArgs<N, T> a)
 {
-  constexpr int CB = gen_cells_per_block<N>();
   constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
-  __shared__ T sQ[CB * N3];
-  __shared__ T sG[3][CB * N3];
-  __shared__ T sF[DIAG ? 1 : CB * 12 * N2];
+  const int CB = a.cb;
+  // dynamic LDS: Q [CB][N3], G [3][CB][N3], F [CB][12][N2] (matvec only)
+  T *sQ = dyn_lds<T>();
+  T *sG[3] = {sQ + CB * N3, sQ + 2 * CB * N3, sQ + 3 * CB * N3};
+  T *sF = sQ + 4 * CB * N3;
   const Tab<N, T> &tb = a.tab;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
-  const long long item0 = (long long)blockIdx.x * CB, item = item0 + c;
+  const long long item0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB, item = item0 + c;
   const bool act = c < CB && item < a.nitems;
   const long long cell = DIAG ? item / N3 : item;
   const int unit = DIAG ? (int)(item % N3) : 0; // diagonal: dof of the unit vector
@@ -265,8 +264,8 @@ __global__ void __launch_bounds__(gen_threads<N>()) sipg_general_kernel(const Ge
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       const int q = k * N2 + l;
-      const T g00 = m[q], g11 = m[N3 + q], g22 = m[2 * N3 + q], g01 = m[3 * N3 + q], g02 = m[4 * N3 + q],
-              g12 = m[5 * N3 + q];
+      const T g00 = load_stream(m + q), g11 = load_stream(m + N3 + q), g22 = load_stream(m + 2 * N3 + q),
+              g01 = load_stream(m + 3 * N3 + q), g02 = load_stream(m + 4 * N3 + q), g12 = load_stream(m + 5 * N3 + q);
       const T x = gx[k], y = gy[k], z = gz[k];
       gx[k] = g00 * x + g01 * y + g02 * z + tb.fv0[k] * tdz[0][0] + tb.fv1[k] * tdz[1][0];
       gy[k] = g01 * x + g11 * y + g12 * z + tb.fv0[k] * tdz[0][1] + tb.fv1[k] * tdz[1][1];
@@ -390,11 +389,16 @@ static hipError_t launch_general_n(const SipgLaunch &L, hipStream_t s)
   a.nx = L.nc[0];
   a.ny = L.nc[1];
   a.nz = L.nc[2];
-  constexpr int CB = gen_cells_per_block<N>();
-  const dim3 grid((unsigned)((a.nitems + CB - 1) / CB)), block(gen_threads<N>());
-  if (L.diagonal) hipLaunchKernelGGL((sipg_general_kernel<N, T, false, true>), grid, block, 0, s, a);
-  else if (L.cheby) hipLaunchKernelGGL((sipg_general_kernel<N, T, true, false>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((sipg_general_kernel<N, T, false, false>), grid, block, 0, s, a);
+  a.cb = L.cb > 0 ? L.cb : gen_cells_per_block<N>();
+  a.order = L.order;
+  if (a.cb * N * N > kGenMaxThreads) return hipErrorInvalidConfiguration;
+  const size_t lds = size_t(a.cb) * (4 * N * N * N + (L.diagonal ? 0 : 12 * N * N)) * sizeof(T);
+  const dim3 grid((unsigned)((a.nitems + a.cb - 1) / a.cb)), block(((a.cb * N * N + 63) / 64) * 64);
+  auto kern = L.diagonal ? sipg_general_kernel<N, T, false, true>
+                         : (L.cheby ? sipg_general_kernel<N, T, true, false> : sipg_general_kernel<N, T, false, false>);
+  const hipError_t e = allow_dynamic_lds((const void *)kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, grid, block, lds, s, a);
   return hipGetLastError();
 }
 

@@ -1,0 +1,170 @@
+// Workgroup schedules of the operator kernels and the autotuner over them
+// (include/sfk.h, "workgroup schedule and autotuner").  The reference picks
+// its loop blocking by hand per degree (legacy/src/test_dg_integrate.cc:64-66);
+// here the schedule of a launch is (cells per workgroup, block order,
+// continuous scatter) and sfk_tune measures every candidate.
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "sfk.h"
+#include "sfk_internal.h"
+#include "sfk_kernels.h"
+#include "sfk_op.h"
+
+using namespace sfk;
+
+namespace {
+
+constexpr int kMaxThreads = 256; // __launch_bounds__ of the operator kernels
+
+#define TUNE_FAIL(code, msg) \
+  do {                       \
+    set_error(msg);          \
+    return code;             \
+  } while (0)
+
+// the kernel's default cells per workgroup (mirrors the device constants)
+int default_cb(const sfk_op *op)
+{
+  const int n = op->d.degree + 1;
+  if (op->d.kind == SFK_DG_SIPG && op->d.geometry == SFK_GEOM_PER_Q) {
+    static const int tab[10] = {0, 0, 64, 28, 12, 8, 5, 3, 2, 1};
+    return tab[n];
+  }
+  return kMaxThreads / (n * n) > 0 ? kMaxThreads / (n * n) : 1;
+}
+
+// LDS bytes per cell of the operator kernel behind sfk_apply
+size_t lds_per_cell(const sfk_op *op)
+{
+  const size_t n = op->d.degree + 1, n2 = n * n, n3 = n2 * n;
+  const size_t es = elem_size(op);
+  switch (op->d.kind) {
+  case SFK_DG_SIPG:
+    return (op->d.geometry == SFK_GEOM_PER_Q ? 4 * n3 + 12 * n2 : 2 * n3 + 12 * n2) * es;
+  case SFK_DG_ADVECT: return (2 * n3 + 6 * n2) * es;
+  default: return 2 * n3 * es;
+  }
+}
+
+int max_cb(const sfk_op *op)
+{
+  const int n = op->d.degree + 1;
+  int dev = 0, lds_max = 65536;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      lds_max = v;
+  }
+  int m = kMaxThreads / (n * n);
+  const int by_lds = (int)(size_t(lds_max) / lds_per_cell(op));
+  if (by_lds < m) m = by_lds;
+  return m > 0 ? m : 1;
+}
+
+} // namespace
+
+extern "C" {
+
+sfk_status sfk_op_get_schedule(const sfk_op *op, sfk_schedule *s)
+{
+  if (!op || !s) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_op_get_schedule: null argument");
+  s->cells_per_block = op->sched_cb > 0 ? op->sched_cb : default_cb(op);
+  s->block_order = op->sched_order;
+  s->cont_scatter = op->cont_scatter;
+  return SFK_OK;
+}
+
+sfk_status sfk_op_max_cells_per_block(const sfk_op *op, int *max_cells)
+{
+  if (!op || !max_cells) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_op_max_cells_per_block: null argument");
+  *max_cells = max_cb(op);
+  return SFK_OK;
+}
+
+sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s)
+{
+  if (!op || !s) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_op_set_schedule: null argument");
+  const int m = max_cb(op);
+  if (s->cells_per_block < 0 || s->cells_per_block > m)
+    TUNE_FAIL(SFK_ERR_INVALID_ARG, "cells_per_block must be in 0.." + std::to_string(m));
+  if (s->block_order != 0 && s->block_order != 1) TUNE_FAIL(SFK_ERR_INVALID_ARG, "block_order must be 0 or 1");
+  if (s->cont_scatter < -1 || s->cont_scatter > 1) TUNE_FAIL(SFK_ERR_INVALID_ARG, "cont_scatter must be -1, 0 or 1");
+  op->sched_cb = s->cells_per_block;
+  op->sched_order = s->block_order;
+  op->cont_scatter = s->cont_scatter;
+  return SFK_OK;
+}
+
+sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_seconds, void *stream)
+{
+  if (!op) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_tune: null handle");
+  if (repeats < 1 || repeats > 10000) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_tune: repeats must be in 1..10000");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t bytes = size_t(op->nvec) * op->d.n_rhs * elem_size(op);
+  void *x = nullptr, *y = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  sfk_status status = SFK_OK;
+  const int saved_cb = op->sched_cb, saved_order = op->sched_order, saved_scatter = op->cont_scatter;
+  sfk_schedule win{saved_cb > 0 ? saved_cb : default_cb(op), saved_order, saved_scatter};
+  double win_t = -1.;
+  if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
+    set_error("sfk_tune: scratch allocation failed");
+    status = SFK_ERR_ALLOC;
+  }
+  if (status == SFK_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+    set_error("sfk_tune: event creation failed");
+    status = SFK_ERR_HIP;
+  }
+  if (status == SFK_OK)
+    status = sfk_vec_fill(op->d.precision, x, op->nvec * op->d.n_rhs, 1.0, stream);
+  if (status == SFK_OK) {
+    const int m = max_cb(op);
+    std::vector<int> scatters = {op->cont_scatter};
+    if (op->d.kind == SFK_CONT_LAPLACE) scatters = {0, 1};
+    for (int sc : scatters)
+      for (int order = 0; order <= 1 && status == SFK_OK; ++order)
+        for (int cb = 1; cb <= m && status == SFK_OK; ++cb) {
+          op->sched_cb = cb;
+          op->sched_order = order;
+          op->cont_scatter = sc;
+          status = sfk_apply(op, x, y, stream); // warm-up (and LDS attribute)
+          if (status != SFK_OK) break;
+          if (hipEventRecord(e0, s) != hipSuccess) status = SFK_ERR_HIP;
+          for (int r = 0; r < repeats && status == SFK_OK; ++r) status = sfk_apply(op, x, y, stream);
+          if (status != SFK_OK) break;
+          float ms = 0.f;
+          if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            set_error("sfk_tune: event timing failed");
+            status = SFK_ERR_HIP;
+            break;
+          }
+          const double t = 1e-3 * ms / repeats;
+          if (win_t < 0. || t < win_t) {
+            win_t = t;
+            win = sfk_schedule{cb, order, sc};
+          }
+        }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (x) (void)hipFree(x);
+  if (y) (void)hipFree(y);
+  if (status != SFK_OK || win_t < 0.) {
+    op->sched_cb = saved_cb;
+    op->sched_order = saved_order;
+    op->cont_scatter = saved_scatter;
+    return status != SFK_OK ? status : SFK_ERR_HIP;
+  }
+  op->sched_cb = win.cells_per_block;
+  op->sched_order = win.block_order;
+  op->cont_scatter = win.cont_scatter;
+  if (best) *best = win;
+  if (best_seconds) *best_seconds = win_t;
+  return SFK_OK;
+}
+
+} // extern "C"

@@ -86,6 +86,27 @@ def test_setters_validate_sizes():
         op2.set_coefficient(np.zeros(8 * 27))  # operator lacks SFK_FLAG_VARIABLE_COEF
 
 
+@pytest.mark.parametrize("degree,kind,geom,default_cb", [
+    (5, sfk.DG_SIPG, sfk.GEOM_PER_Q, 5), (5, sfk.DG_SIPG, sfk.GEOM_CARTESIAN, 7),
+    (4, sfk.CONT_LAPLACE, sfk.GEOM_CARTESIAN, 10), (8, sfk.CELL_LAPLACE, sfk.GEOM_CARTESIAN, 3),
+    (3, sfk.DG_ADVECT, sfk.GEOM_CARTESIAN, 16)])
+def test_schedule_defaults_and_validation(degree, kind, geom, default_cb):
+    kw = {"basis": sfk.BASIS_GLL} if kind in (sfk.CONT_LAPLACE, sfk.DG_ADVECT) else {}
+    if kind == sfk.DG_ADVECT:
+        kw["velocity"] = (1.0, 0.15, -0.05)
+    op = sfk.Operator(kind, degree, (4, 4, 4), geometry=geom, inv_jac=(4, 4, 4), **kw)
+    assert op.schedule == (default_cb, 1, -1)
+    m = op.max_cells_per_block
+    assert default_cb <= m <= 256 // (degree + 1) ** 2
+    op.set_schedule(1, 0, 1)
+    assert op.schedule == (1, 0, 1)
+    for bad in [(m + 1, 1, -1), (-1, 1, -1), (0, 2, -1), (0, 1, 2)]:
+        with pytest.raises(sfk.SfkError):
+            op.set_schedule(*bad)
+    with pytest.raises(sfk.SfkError):
+        op.tune(repeats=0, stream=0)  # rejected before any device work
+
+
 @pytest.mark.parametrize("degree", range(1, 9))
 @pytest.mark.parametrize("basis", [0, 1])
 def test_library_tables_equal_oracle(degree, basis):

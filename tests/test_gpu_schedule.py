@@ -1,0 +1,116 @@
+"""Workgroup schedules and the autotuner: every schedule of an operator
+kernel (cells per workgroup, block order, continuous scatter) gives the
+oracle's result, and DG/element-local results do not depend on the schedule
+at all (same arithmetic per cell, bit-identical)."""
+import numpy as np
+import pytest
+
+import oracle_bindings as ob
+from helpers import TOL_FP64, rel_err, sine_metric, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+sfk = pytest.importorskip("sum_factorization_kernels_amd")
+
+
+def _schedules(op, scatters=(-1,)):
+    m = op.max_cells_per_block
+    cbs = sorted({1, 2, max(1, m // 2), m})
+    return [(cb, order, sc) for sc in scatters for order in (0, 1) for cb in cbs]
+
+
+@pytest.mark.parametrize("degree", [1, 2, 4, 6, 8])
+def test_cont_schedules_match_oracle(cuda, degree):
+    # odd cell counts: partial segments and single-row colours
+    nc = (7, 3, 5) if degree <= 4 else (5, 3, 3)
+    ij = tuple(float(c) for c in nc)
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    rng = np.random.default_rng(4000 + degree)
+    src = rng.uniform(-1, 1, op.vector_size)
+    ref = ob.cont_laplacian(degree, nc, ij, src)
+    d_src = to_dev(src)
+    colored = {}
+    for sched in _schedules(op, scatters=(0, 1)):
+        op.set_schedule(*sched)
+        assert op.schedule == sched
+        out = to_np(op.apply(d_src))
+        assert rel_err(out, ref) < TOL_FP64, sched
+        if sched[2] == 1:
+            # colored stores: the summation order of a node depends on the
+            # segment length only, not on the block order
+            cb = sched[0]
+            if cb in colored:
+                assert np.array_equal(out, colored[cb]), sched
+            else:
+                colored[cb] = out
+            # repeated applies are deterministic
+            assert np.array_equal(to_np(op.apply(d_src)), out)
+
+
+@pytest.mark.parametrize("degree", [2, 5])
+def test_cont_colored_multi_rhs_coefficient(cuda, degree):
+    nc = (5, 4, 3)
+    ij = (5.0, 4.0, 3.0)
+    n3 = (degree + 1) ** 3
+    rng = np.random.default_rng(4100 + degree)
+    k = np.exp(0.5 * rng.standard_normal(int(np.prod(nc)) * n3))
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij, flags=sfk.FLAG_VARIABLE_COEF, n_rhs=2)
+    op.set_coefficient(k)
+    op.set_schedule(2, 1, 1)
+    N = op.vector_size
+    src = rng.standard_normal(2 * N)
+    out = to_np(op.apply(to_dev(src)))
+    for r in range(2):
+        ref = ob.cont_laplacian(degree, nc, ij, src[r * N:(r + 1) * N].copy(), coef=k)
+        assert rel_err(out[r * N:(r + 1) * N], ref) < TOL_FP64
+
+
+@pytest.mark.parametrize("degree", [1, 3, 5, 8])
+def test_dg_schedules_bit_identical(cuda, degree):
+    nc = (5, 3, 4)
+    ij = (4.0, 2.0, 4.0 / 3.0)
+    n3 = (degree + 1) ** 3
+    rng = np.random.default_rng(4200 + degree)
+    src = rng.standard_normal(n3 * int(np.prod(nc)))
+    d_src = to_dev(src)
+    cases = [
+        (sfk.Operator(sfk.DG_SIPG, degree, nc, inv_jac=ij), None),
+        (sfk.Operator(sfk.CELL_LAPLACE, degree, (37, 1, 1), inv_jac=ij), None),
+        (sfk.Operator(sfk.DG_ADVECT, degree, nc, inv_jac=ij, velocity=(1.0, 0.15, -0.05)), None),
+    ]
+    gen = sfk.Operator(sfk.DG_SIPG, degree, nc, geometry=sfk.GEOM_PER_Q)
+    cm, fm = sine_metric(degree, nc, 0.1)
+    gen.set_cell_metric(cm)
+    gen.set_face_metric(fm)
+    cases.append((gen, ob.dg_sipg_general(degree, nc, cm, fm, src)))
+    for op, ref in cases:
+        x = d_src if op.vector_size == len(src) else to_dev(rng.standard_normal(op.vector_size))
+        base = to_np(op.apply(x))
+        if ref is not None:
+            assert rel_err(base, ref) < TOL_FP64
+        for sched in _schedules(op):
+            op.set_schedule(*sched)
+            assert np.array_equal(to_np(op.apply(x)), base), (op.kind, sched)
+
+
+def test_schedule_validation(cuda):
+    op = sfk.Operator(sfk.DG_SIPG, 5, (4, 4, 4), geometry=sfk.GEOM_PER_Q)
+    m = op.max_cells_per_block
+    assert 1 <= m <= 256 // 36
+    for bad in [(m + 1, 1, -1), (-1, 1, -1), (0, 2, -1), (0, 1, 2)]:
+        with pytest.raises(sfk.SfkError):
+            op.set_schedule(*bad)
+    op.set_schedule(0, 0, -1)
+    assert op.schedule[0] == 5  # the degree's default
+
+
+def test_tune_picks_valid_schedule(cuda):
+    degree, nc = 4, (6, 5, 4)
+    ij = (6.0, 5.0, 4.0)
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    sched, t = op.tune(repeats=3)
+    assert t > 0
+    assert 1 <= sched[0] <= op.max_cells_per_block and sched[1] in (0, 1) and sched[2] in (0, 1)
+    assert op.schedule == sched
+    src = np.random.default_rng(4300).uniform(-1, 1, op.vector_size)
+    assert rel_err(to_np(op.apply(to_dev(src))), ob.cont_laplacian(degree, nc, ij, src)) < TOL_FP64
