@@ -106,6 +106,9 @@ hipError_t vec_copy_convert(bool to_fp32, const void *x, void *y, long long n, h
 hipError_t vec_dot(bool fp32, const void *x, const void *y, long long n, double *result_dev,
                    double *partials, int n_partials, hipStream_t s);
 // y = a*x + b*y (FP64 scalars from device memory if ptrs given)
+// result_dev[0] = x.y, result_dev[1] = x.w (FP64 vectors; partials: 2*n_partials)
+hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, double *result_dev, double *partials,
+                    int n_partials, hipStream_t s);
 hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, void *y, hipStream_t s);
 hipError_t vec_cheby_update(bool fp32, long long n, const void *au, const void *rhs, const void *sol_old,
                             const void *diag, void *sol_new, void *tm, double c_np, double c_tm,

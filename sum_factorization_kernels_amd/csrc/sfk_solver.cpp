@@ -1,14 +1,11 @@
 // Chebyshev-preconditioned CG for the continuous Q_p Laplacian, all on the
 // device (the reference has no solver; north-star extension).
 //
-//   FP64 smoother:  PCG in FP64 with a Chebyshev(k) point-Jacobi polynomial
-//                   preconditioner, to ||b - A x|| <= tol ||b||.
-//   FP32 smoother:  defect correction in FP64 (r = b - A x with the FP64
-//                   operator, x += e) around an inner flexible PCG that runs
-//                   entirely on the FP32 operator with FP32 vectors (inner
-//                   products accumulated in FP64) and the FP32 Chebyshev
-//                   preconditioner; each inner solve reduces its residual by
-//                   kInnerReduction.
+// PCG in FP64 on `op` (iterate, residual, search direction, inner products),
+// preconditioned by a Chebyshev(k) point-Jacobi polynomial applied on the
+// smoother operator: FP64, or FP32 with FP32 vectors (the residual is
+// rounded to FP32 on entry and the correction widened to FP64 on exit), so
+// k-1 of the k matvecs per iteration run in the smoother's precision.
 // Chebyshev bounds: lambda_max of D^{-1}A from `lanczos_steps` FP64 Lanczos
 // steps on D^{-1/2} A D^{-1/2} (largest Ritz value by bisection on the
 // tridiagonal matrix), times 1.1; lambda_min = lambda_max / 30.
@@ -31,7 +28,6 @@
 namespace sfk {
 namespace {
 
-constexpr double kInnerReduction = 1e-4;
 constexpr int kPartials = 2048;
 
 struct DevBuf {
@@ -125,6 +121,26 @@ struct Solver {
     }
     return SFK_OK;
   }
+
+  // (x . y, x . w) in one pass, FP64 vectors
+  sfk_status dot2(const void *x, const void *y, const void *w, double &xy, double &xw)
+  {
+    hipError_t e = vec_dot2(x, y, w, n, dres, partials, kPartials, s);
+    double h[2] = {0., 0.};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, dres, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error(std::string("solver: ") + hipGetErrorString(e));
+      return SFK_ERR_HIP;
+    }
+    xy = h[0];
+    xw = h[1];
+    if (!std::isfinite(xy) || !std::isfinite(xw)) {
+      set_error("solver: non-finite inner product");
+      return SFK_ERR_NONFINITE;
+    }
+    return SFK_OK;
+  }
 };
 
 #define SOLVER_HIP(expr)                                              \
@@ -174,20 +190,20 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   const size_t n8 = (size_t)n * 8, ns = (size_t)n * (f32 ? 4 : 8);
   // device work space: FP64 vectors, and smoother-precision vectors
   DevBuf red, d64, inv64, isq, r, w, lv0, lv1, lw, lt, e64;
-  DevBuf rs, es, zs, ps, qs, zero, tms, rold, inv_s;
-  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (kPartials + 1)));
+  DevBuf rs, zs, qs, zero, tms, inv_s;
+  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (2 * kPartials + 2)));
   S.partials = (double *)red.p;
-  S.dres = S.partials + kPartials;
+  S.dres = S.partials + 2 * kPartials; // after the partials of two sums: two result slots
   for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
     SOLVER_HIP(hipMalloc(&bb->p, n8));
-  for (DevBuf *bb : {&rs, &es, &zs, &ps, &qs, &zero, &tms, &rold, &inv_s})
+  for (DevBuf *bb : {&rs, &zs, &qs, &zero, &tms, &inv_s})
     SOLVER_HIP(hipMalloc(&bb->p, ns));
   const hipStream_t s = S.s;
   // every work vector starts at 0 (a fresh allocation may hold NaN bit
   // patterns, and 0 * NaN would poison the first Chebyshev step)
   for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
     SOLVER_HIP(vec_fill(false, bb->p, n, 0., s));
-  for (DevBuf *bb : {&rs, &es, &zs, &ps, &qs, &zero, &tms, &rold, &inv_s})
+  for (DevBuf *bb : {&rs, &zs, &qs, &zero, &tms, &inv_s})
     SOLVER_HIP(vec_fill(f32, bb->p, n, 0., s));
   // point-Jacobi diagonal (FP64), inverse in both precisions
   SOLVER_ST(sfk_compute_diagonal(op, d64.p, stream));
@@ -266,71 +282,57 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   double res = S.st.initial_residual;
   int iters = 0;
 
-  // flexible PCG for A e = r (r = the FP64 residual in r.p), e = 0 at start,
-  // until ||r - A e|| <= target (absolute) or `budget` iterations; runs in
-  // the smoother's precision on the smoother operator (FP64 case: op)
-  auto pcg = [&](double target, int budget, int &used) -> sfk_status {
-    const size_t vb = ns;
-    void *rv = f32 ? rs.p : r.p;
-    if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rv, n, s));
-    void *e = f32 ? es.p : w.p;
-    SOLVER_HIP(vec_fill(f32, e, n, 0., s));
-    void *z = zs.p, *spare = qs.p;
-    SOLVER_ST(cheby(rv, &z, &spare));
-    SOLVER_HIP(hipMemcpyAsync(ps.p, z, vb, hipMemcpyDeviceToDevice, s));
-    double rz;
-    SOLVER_ST(S.dot(f32, rv, z, rz));
-    used = 0;
-    while (used < budget) {
-      void *qv = spare; // neither z nor p
-      SOLVER_ST(sfk_apply(S.sm, ps.p, qv, stream));
-      if (f32) ++S.st.matvecs_inner;
-      else ++S.st.matvecs_outer;
-      double pq;
-      SOLVER_ST(S.dot(f32, ps.p, qv, pq));
-      const double alpha = rz / pq;
-      SOLVER_HIP(vec_axpby(f32, n, alpha, ps.p, 1., e, s));
-      SOLVER_HIP(hipMemcpyAsync(rold.p, rv, vb, hipMemcpyDeviceToDevice, s));
-      SOLVER_HIP(vec_axpby(f32, n, -alpha, qv, 1., rv, s));
-      ++used;
-      double rn;
-      SOLVER_ST(S.dot(f32, rv, rv, rn));
-      if (std::sqrt(rn) <= target) break;
-      SOLVER_ST(cheby(rv, &z, &spare));
-      double zr, zro;
-      SOLVER_ST(S.dot(f32, z, rv, zr));
-      SOLVER_ST(S.dot(f32, z, rold.p, zro));
-      const double beta_cg = (zr - zro) / rz; // flexible (Polak-Ribiere) coefficient
-      rz = zr;
-      SOLVER_HIP(vec_axpby(f32, n, 1., z, beta_cg, ps.p, s));
+  // preconditioner z = P(r) (FP64 in and out): the Chebyshev polynomial on
+  // the smoother operator, in the smoother's precision; *z64 receives the
+  // FP64 buffer that holds the result
+  auto precond = [&](const void *r64, void **z64) -> sfk_status {
+    if (f32) {
+      void *z = zs.p, *spare = qs.p;
+      SOLVER_HIP(vec_copy_convert(true, r64, rs.p, n, s));
+      SOLVER_ST(cheby(rs.p, &z, &spare));
+      SOLVER_HIP(vec_copy_convert(false, z, e64.p, n, s));
+      *z64 = e64.p;
+    } else {
+      void *z = e64.p, *spare = zs.p;
+      SOLVER_ST(cheby(r64, &z, &spare));
+      *z64 = z;
     }
     return SFK_OK;
   };
 
+  // PCG in FP64 (x, r, p, q, z and all inner products FP64).  The FP32
+  // polynomial preconditioner is linear only up to FP32 rounding, so the
+  // search direction uses the flexible (Polak-Ribiere) coefficient
+  // beta = z_new . (r_new - r_old) / (z_old . r_old) = -alpha (z_new . q) / rz.
   const double target = rel_tol * bnorm;
-  if (!f32) {
-    // all FP64: one PCG to the final tolerance
-    if (res > target) {
-      int used = 0;
-      SOLVER_ST(pcg(target, max_iter, used));
-      iters = used;
-      SOLVER_HIP(vec_axpby(false, n, 1., w.p, 1., x, s));
-    }
-  } else {
-    // defect correction: FP64 residual and update, FP32 inner PCG
-    int outer = 0;
-    while (res > target && iters < max_iter && outer < 60) {
-      int used = 0;
-      SOLVER_ST(pcg(std::max(kInnerReduction * res, 0.5 * target), max_iter - iters, used));
-      iters += used;
-      ++outer;
-      SOLVER_HIP(vec_copy_convert(false, es.p, e64.p, n, s));
-      SOLVER_HIP(vec_axpby(false, n, 1., e64.p, 1., x, s));
-      SOLVER_ST(sfk_apply(op, x, r.p, stream));
+  if (res > target) {
+    void *z64 = nullptr, *pv = w.p, *qv = lt.p;
+    SOLVER_ST(precond(r.p, &z64));
+    SOLVER_HIP(hipMemcpyAsync(pv, z64, n8, hipMemcpyDeviceToDevice, s));
+    double rz;
+    SOLVER_ST(S.dot(false, r.p, z64, rz));
+    while (iters < max_iter) {
+      SOLVER_ST(sfk_apply(op, pv, qv, stream));
       ++S.st.matvecs_outer;
-      SOLVER_HIP(vec_axpby(false, n, 1., b, -1., r.p, s));
+      double pq;
+      SOLVER_ST(S.dot(false, pv, qv, pq));
+      if (!(pq > 0.)) {
+        set_error("sfk_solve_cg_cheby: operator not positive definite on the search direction");
+        return SFK_ERR_NONFINITE;
+      }
+      const double alpha = rz / pq;
+      SOLVER_HIP(vec_axpby(false, n, alpha, pv, 1., x, s));
+      SOLVER_HIP(vec_axpby(false, n, -alpha, qv, 1., r.p, s));
+      ++iters;
       SOLVER_ST(S.dot(false, r.p, r.p, rr));
       res = std::sqrt(rr);
+      if (res <= target) break;
+      SOLVER_ST(precond(r.p, &z64));
+      double zr, zq;
+      SOLVER_ST(S.dot2(z64, r.p, qv, zr, zq));
+      const double beta_cg = -alpha * zq / rz;
+      rz = zr;
+      SOLVER_HIP(vec_axpby(false, n, 1., z64, beta_cg, pv, s));
     }
   }
   // final FP64 residual

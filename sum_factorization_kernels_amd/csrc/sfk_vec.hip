@@ -64,6 +64,32 @@ __global__ void __launch_bounds__(kThreads) dot_final_kernel(const double *parti
   if (threadIdx.x == 0) *result = red[0];
 }
 
+__global__ void __launch_bounds__(kThreads) dot2_partial_kernel(const double *x, const double *y, const double *w,
+                                                                long long n, double *partials)
+{
+  __shared__ double red[2][kThreads];
+  double s = 0., u = 0.;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double xv = x[i];
+    s += xv * y[i];
+    u += xv * w[i];
+  }
+  red[0][threadIdx.x] = s;
+  red[1][threadIdx.x] = u;
+  __syncthreads();
+  for (int k = kThreads / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + k];
+      red[1][threadIdx.x] += red[1][threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = red[0][0];
+    partials[gridDim.x + blockIdx.x] = red[1][0];
+  }
+}
+
 template <typename T>
 __global__ void axpby_kernel(long long n, T a, const T *x, T b, T *y)
 {
@@ -156,6 +182,19 @@ hipError_t vec_dot(bool fp32, const void *x, const void *y, long long n, double 
   else
     hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(g), dim3(kThreads), 0, s, (const double *)x, (const double *)y, n, partials);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
+  return hipGetLastError();
+}
+
+hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, double *result_dev, double *partials,
+                    int n_partials, hipStream_t s)
+{
+  unsigned g = grid_for(n, 8);
+  if ((int)g > n_partials) g = n_partials;
+  hipLaunchKernelGGL(dot2_partial_kernel, dim3(g), dim3(kThreads), 0, s, (const double *)x, (const double *)y,
+                     (const double *)w, n, partials);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials + g, (int)g,
+                     result_dev + 1);
   return hipGetLastError();
 }
 
