@@ -113,6 +113,13 @@ sfk_status sfk_op_set_velocity(sfk_op *op, const double *cell_host, size_t cell_
  * of the uniform mesh of [0,1]^3 on the device (setup, not timed).
  * Penalty: (p+1)^2 * |F| / |K| averaged over both sides. */
 sfk_status sfk_op_set_sine_deformation(sfk_op *op, double amp);
+/* Device-side forms of sfk_op_set_coefficient / sfk_op_set_velocity: the
+ * arrays are device memory in the handle's precision (same layouts and
+ * counts), copied stream-ordered on `stream` (for callers whose
+ * coefficients live on the GPU, e.g. inside an autodiff graph). */
+sfk_status sfk_op_set_coefficient_device(sfk_op *op, const void *dev, size_t count, void *stream);
+sfk_status sfk_op_set_velocity_device(sfk_op *op, const void *cell_dev, size_t cell_count,
+                                      const void *face_dev, size_t face_count, void *stream);
 
 /* dst = A src (n_rhs vectors).  Replaces matrix_vector_product()
  * (evaluation_cell_laplacian.h:216), do_matvec() (evaluation_dg_laplacian.h
@@ -139,8 +146,10 @@ sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream);
  * advection: with r = A u - f, J = 0.5 |r|^2, writes dJ/d(velocity) into
  *   grad_cell [cell][3][q] and grad_face [d][cell][q] (layouts of
  *   sfk_op_set_velocity); lambda = A u - f is passed in by the caller;
- * variable-coefficient Laplacian: g_q = -JxW_q grad u_q . grad lambda_q
- *   summed over the n_rhs pairs, i.e. d(lambda^T A(k) u)/dk_q. */
+ * variable-coefficient Laplacian: g_q = JxW_q grad u_q . grad lambda_q
+ *   (JxW_q including the quadrature weight) summed over the n_rhs pairs,
+ *   i.e. d(lambda^T A(k) u)/dk_q; for an objective J(u) with A(k) u = f and
+ *   the adjoint solution lambda of A lambda = dJ/du, dJ/dk_q = -g_q. */
 sfk_status sfk_grad_velocity(sfk_op *op, const void *u, const void *lambda, void *grad_cell,
                              void *grad_face, void *stream);
 sfk_status sfk_grad_coefficient(sfk_op *op, const void *u, const void *lambda, void *grad_k,

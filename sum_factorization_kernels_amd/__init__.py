@@ -76,6 +76,8 @@ def lib():
         for name in ("sfk_op_set_cell_metric", "sfk_op_set_face_metric", "sfk_op_set_coefficient"):
             getattr(L, name).argtypes = [_vp, _dp, ctypes.c_size_t]
         L.sfk_op_set_velocity.argtypes = [_vp, _dp, ctypes.c_size_t, _dp, ctypes.c_size_t]
+        L.sfk_op_set_coefficient_device.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
+        L.sfk_op_set_velocity_device.argtypes = [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _vp]
         L.sfk_op_set_sine_deformation.argtypes = [_vp, ctypes.c_double]
         L.sfk_apply.argtypes = [_vp, _vp, _vp, _vp]
         L.sfk_apply_transpose.argtypes = [_vp, _vp, _vp, _vp]
@@ -205,6 +207,22 @@ class Operator:
 
     def set_velocity(self, cell, face):
         check(lib().sfk_op_set_velocity(self.handle, _np_ptr(cell), cell.size, _np_ptr(face), face.size))
+
+    def _check_data(self, t, n):
+        if not (t.is_cuda and t.is_contiguous() and t.dtype == self.dtype and t.numel() == n):
+            raise ValueError(f"expected a contiguous CUDA tensor of {n} {self.dtype} values")
+
+    def set_coefficient_device(self, k, stream=None):
+        """k: device tensor [cell][q] in the operator's precision"""
+        self._check_data(k, self.n_cell_q)
+        check(lib().sfk_op_set_coefficient_device(self.handle, _dptr(k), k.numel(), _stream(stream)))
+
+    def set_velocity_device(self, cell, face, stream=None):
+        """cell: [cell][3][q], face: [d][cell][q] device tensors in the operator's precision"""
+        self._check_data(cell, 3 * self.n_cell_q)
+        self._check_data(face, 3 * self.n_face_q)
+        check(lib().sfk_op_set_velocity_device(self.handle, _dptr(cell), cell.numel(), _dptr(face), face.numel(),
+                                               _stream(stream)))
 
     def set_sine_deformation(self, amp):
         check(lib().sfk_op_set_sine_deformation(self.handle, float(amp)))

@@ -224,6 +224,43 @@ sfk_status sfk_op_set_velocity(sfk_op *op, const double *cell_host, size_t cell_
   return upload(op, &op->face_vel, face_host, face_count, (size_t)op->nfq * 3, "face velocity");
 }
 
+// device-to-device copy into handle-owned storage (allocated on first use)
+static sfk_status copy_device(sfk_op *op, void **dev, const void *src, size_t count, size_t expected,
+                              const char *what, hipStream_t s)
+{
+  SFK_REQUIRE(src != nullptr, SFK_ERR_INVALID_ARG, std::string(what) + ": null device pointer");
+  SFK_REQUIRE(count == expected, SFK_ERR_INVALID_ARG,
+              std::string(what) + ": expected " + std::to_string(expected) + " values, got " + std::to_string(count));
+  const size_t bytes = count * elem_size(op);
+  if (*dev == nullptr && hipMalloc(dev, bytes) != hipSuccess) {
+    *dev = nullptr;
+    set_error(std::string(what) + ": hipMalloc failed");
+    return SFK_ERR_ALLOC;
+  }
+  SFK_CHECK_HIP(hipMemcpyAsync(*dev, src, bytes, hipMemcpyDeviceToDevice, s));
+  return SFK_OK;
+}
+
+sfk_status sfk_op_set_coefficient_device(sfk_op *op, const void *dev, size_t count, void *stream)
+{
+  SFK_REQUIRE(op, SFK_ERR_INVALID_ARG, "null handle");
+  SFK_REQUIRE(op->d.flags & SFK_FLAG_VARIABLE_COEF, SFK_ERR_INVALID_ARG, "operator lacks SFK_FLAG_VARIABLE_COEF");
+  return copy_device(op, &op->coef, dev, count, (size_t)op->ncq, "coefficient", (hipStream_t)stream);
+}
+
+sfk_status sfk_op_set_velocity_device(sfk_op *op, const void *cell_dev, size_t cell_count, const void *face_dev,
+                                      size_t face_count, void *stream)
+{
+  SFK_REQUIRE(op, SFK_ERR_INVALID_ARG, "null handle");
+  SFK_REQUIRE(op->d.flags & SFK_FLAG_VARIABLE_VELOCITY, SFK_ERR_INVALID_ARG,
+              "operator lacks SFK_FLAG_VARIABLE_VELOCITY");
+  sfk_status st = copy_device(op, &op->cell_vel, cell_dev, cell_count, (size_t)op->ncq * 3, "cell velocity",
+                              (hipStream_t)stream);
+  if (st != SFK_OK) return st;
+  return copy_device(op, &op->face_vel, face_dev, face_count, (size_t)op->nfq * 3, "face velocity",
+                     (hipStream_t)stream);
+}
+
 sfk_status sfk_op_set_sine_deformation(sfk_op *op, double amp)
 {
   SFK_REQUIRE(op, SFK_ERR_INVALID_ARG, "null handle");

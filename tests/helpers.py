@@ -99,3 +99,27 @@ def sine_metric(degree, nc, amp):
         fm[d, :, 6, :] = jxwf
         fm[d, :, 7, :] = tau[:, None] * jxwf
     return cm, np.ascontiguousarray(fm).ravel()
+
+
+def advect_velocity_field(degree, nc):
+    """a(x) = (1, 0.5 + 0.25 sin(2 pi x), 0.25) on the uniform periodic mesh
+    of [0,1]^3 at the cell Gauss points ([cell][3][q]) and as a.n on the
+    upper face of every cell in each direction ([d][cell][fa], fa = hi*n +
+    lo over the two in-face directions, lo the faster one)."""
+    import oracle_bindings as ob
+    n = degree + 1
+    xg = ob.gauss_points(n)
+    ncell = int(np.prod(nc))
+    cx = np.arange(ncell) % nc[0]
+    # x coordinate of the Gauss points along x of every cell: (ncell, n)
+    x = (cx[:, None] + xg[None, :]) / nc[0]
+    ay = 0.5 + 0.25 * np.sin(2 * np.pi * x)  # (ncell, n): depends on the x index only
+    cell = np.empty((ncell, 3, n, n, n))  # [cell][comp][k][j][i]
+    cell[:, 0] = 1.0
+    cell[:, 1] = ay[:, None, None, :]
+    cell[:, 2] = 0.25
+    face = np.empty((3, ncell, n, n))  # [d][cell][hi][lo]
+    face[0] = 1.0
+    face[1] = ay[:, None, :]  # d = 1: lo = x, hi = z
+    face[2] = 0.25
+    return cell.ravel(), face.ravel()
