@@ -211,6 +211,35 @@ sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const double *b, 
                               double rel_tol, int max_iter, int cheby_degree, int lanczos_steps,
                               sfk_solve_stats *stats, void *stream);
 
+/* ---- Chebyshev smoother sweep -----------------------------------------
+ * `degree` fused Chebyshev steps (do_cheby_iter<true>,
+ * evaluation_dg_laplacian.h:485-502, :512-554) with the coefficients of the
+ * Chebyshev iteration for D^{-1}A on [lambda_min, lambda_max] (Saad,
+ * Iterative Methods for Sparse Linear Systems, Alg. 12.1):
+ *   step 0: tmp = (1/theta) D^{-1}(A x - rhs), x -= tmp
+ *   step j: tmp = rho_j rho_{j-1} tmp + (2 rho_j/delta) D^{-1}(A x - rhs), x -= tmp
+ * with theta, delta the centre and half-width of the interval.  x is updated
+ * in place (spare: a second vector of the same size; tmp: work vector);
+ * diag_inv = D^{-1}.  DG-SIPG and the Laplacians. */
+sfk_status sfk_cheby_sweep(sfk_op *op, const void *rhs, void *x, void *spare, const void *diag_inv,
+                           void *tmp, int degree, double lambda_min, double lambda_max, void *stream);
+
+/* ---- HIP graphs --------------------------------------------------------
+ * Captures the sfk calls issued on a (non-default) stream between
+ * sfk_graph_begin and sfk_graph_end into an executable graph that
+ * sfk_graph_launch replays with the same pointers and scalars (e.g. a
+ * Chebyshev sweep or the five stages of an LSRK time step).  Issue every
+ * call once before capturing: an operator allocates its scratch space on
+ * first use.  Graph nodes are kernels and device copies only (no memset
+ * nodes). */
+typedef struct sfk_graph sfk_graph;
+sfk_status sfk_stream_create(void **stream); /* default priority */
+sfk_status sfk_stream_destroy(void *stream);
+sfk_status sfk_graph_begin(void *stream);
+sfk_status sfk_graph_end(void *stream, sfk_graph **graph);
+sfk_status sfk_graph_launch(const sfk_graph *graph, void *stream);
+sfk_status sfk_graph_destroy(sfk_graph *graph);
+
 /* ---- 1D kernel (a standalone batched form of apply_1d_matvec_kernel) ---
  * Applies the 1D even-odd kernel to n_lines lines of nn values, line l at
  * in + l*line_stride with element stride `stride` (device FP64 arrays);

@@ -102,6 +102,14 @@ def lib():
         L.sfk_op_set_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
         L.sfk_op_max_cells_per_block.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
         L.sfk_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Schedule), _dp, _vp]
+        L.sfk_cheby_sweep.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, _vp]
+        L.sfk_stream_create.argtypes = [ctypes.POINTER(_vp)]
+        L.sfk_stream_destroy.argtypes = [_vp]
+        L.sfk_graph_begin.argtypes = [_vp]
+        L.sfk_graph_end.argtypes = [_vp, ctypes.POINTER(_vp)]
+        L.sfk_graph_launch.argtypes = [_vp, _vp]
+        L.sfk_graph_destroy.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -299,6 +307,13 @@ class Operator:
         check(lib().sfk_tune(self.handle, int(repeats), ctypes.byref(sc), ctypes.byref(t), _stream(stream)))
         return (sc.cells_per_block, sc.block_order, sc.cont_scatter), t.value
 
+    def cheby_sweep(self, rhs, x, spare, diag_inv, tmp, degree, lambda_min, lambda_max, stream=None):
+        """`degree` fused Chebyshev steps on [lambda_min, lambda_max]; x updated in place"""
+        for t in (rhs, x, spare, diag_inv, tmp):
+            self._check_vec(t)
+        check(lib().sfk_cheby_sweep(self.handle, _dptr(rhs), _dptr(x), _dptr(spare), _dptr(diag_inv), _dptr(tmp),
+                                    int(degree), float(lambda_min), float(lambda_max), _stream(stream)))
+
     def grad_coefficient(self, u, lam, stream=None):
         import torch
         self._check_vec(u)
@@ -306,6 +321,57 @@ class Operator:
         g = torch.zeros(self.n_cell_q, dtype=self.dtype, device="cuda")
         check(lib().sfk_grad_coefficient(self.handle, _dptr(u), _dptr(lam), _dptr(g), _stream(stream)))
         return g
+
+
+class Stream:
+    """A non-default HIP stream (default priority) owned by the library."""
+
+    def __init__(self):
+        self.handle = _vp()
+        check(lib().sfk_stream_create(ctypes.byref(self.handle)))
+
+    @property
+    def ptr(self):
+        return self.handle.value
+
+    def synchronize(self):
+        check(lib().sfk_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().sfk_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Graph:
+    """sfk calls issued on `stream` inside `with Graph(stream) as g:` are
+    captured; g.launch() replays them."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.handle = _vp()
+
+    def __enter__(self):
+        check(lib().sfk_graph_begin(self.stream.handle))
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        status = lib().sfk_graph_end(self.stream.handle, ctypes.byref(self.handle))
+        if exc_type is None:
+            check(status)
+        return False
+
+    def launch(self):
+        check(lib().sfk_graph_launch(self.handle, self.stream.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().sfk_graph_destroy(self.handle)
+        except Exception:
+            pass
 
 
 def solve_cg_cheby(op, smoother, b, x, rel_tol=1e-10, max_iter=1000, cheby_degree=6, lanczos_steps=12,
