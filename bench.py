@@ -59,20 +59,6 @@ def sipg_bytes_per_dof(degree, elem):
     return elem * (2 + 6 + 3 * 8 * n * n / n ** 3)
 
 
-def chebyshev_coefficients(lmax, lmin, degree):
-    """(c_np, c_tm) of the fused steps 1..degree-1 of the Chebyshev recurrence
-    (Saad, Alg. 12.1), step 0 being c_np = 1/theta, c_tm = 0"""
-    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
-    sigma = theta / delta
-    out = [(1.0 / theta, 0.0)]
-    rho_old = 1.0 / sigma
-    for _ in range(1, degree):
-        rho = 1.0 / (2.0 * sigma - rho_old)
-        out.append((2.0 * rho / delta, rho * rho_old))
-        rho_old = rho
-    return out
-
-
 def power_lmax(op, dinv, n, iters=15):
     import torch
     v = torch.randn(n, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(7))
@@ -166,6 +152,10 @@ def main():
     g = torch.Generator("cuda").manual_seed(1)
     u = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
     v = torch.empty_like(u)
+    # workgroup schedule (cells per workgroup, block order) chosen by the
+    # library's autotuner before the timed region
+    sched, _ = op.tune(repeats=10)
+    log(f"autotuned schedule (cells_per_block, block_order, scatter): {sched}")
     t64 = timed(lambda: op.apply(u, v), args.steps, args.warmup)
     gdofs = N / t64 / 1e9
     bytes_mv = sipg_bytes_per_dof(degree, 8) * N
@@ -174,6 +164,8 @@ def main():
     if not args.only_headline:
         op32 = sfk.Operator(sfk.DG_SIPG, degree, nc, geometry=sfk.GEOM_PER_Q, precision=sfk.FP32)
         op32.set_sine_deformation(0.1)
+        sched32, _ = op32.tune(repeats=10)
+        extra["sipg_p5_deformed_fp32_schedule"] = list(sched32)
         u32, v32 = u.float(), torch.empty(N, dtype=torch.float32, device="cuda")
         t32 = timed(lambda: op32.apply(u32, v32), args.steps, args.warmup)
         extra["sipg_p5_deformed_fp32_gdofs"] = round(N / t32 / 1e9, 3)
@@ -183,19 +175,27 @@ def main():
         diag = op.diagonal()
         dinv = 1.0 / diag
         lmax = 1.1 * power_lmax(op, dinv, N)
-        coefs = chebyshev_coefficients(lmax, lmax / 30.0, 5)
         rhs = torch.ones(N, dtype=torch.float64, device="cuda")
         xa, xb, tm = torch.zeros_like(u), torch.empty_like(u), torch.zeros_like(u)
-
-        def sweep():
-            src, dst = xa, xb
-            for c_np, c_tm in coefs:
-                op.cheby_step(rhs, src, dinv, dst, tm, c_np, c_tm)
-                src, dst = dst, src
-        tch = timed(sweep, max(10, args.steps // 10), 3)
+        tch = timed(lambda: op.cheby_sweep(rhs, xa, xb, dinv, tm, 5, lmax / 30.0, lmax),
+                    max(10, args.steps // 10), 3)
         extra["sipg_p5_deformed_cheby5_sweep_ms"] = round(tch * 1e3, 4)
         extra["sipg_p5_deformed_cheby_step_gdofs"] = round(5 * N / tch / 1e9, 3)
         log(f"Chebyshev(5) sweep: {tch * 1e3:.3f} ms")
+        # the same sweep captured once in a HIP graph and replayed
+        st = sfk.Stream()
+        torch.cuda.synchronize()
+        with sfk.Graph(st) as graph:
+            op.cheby_sweep(rhs, xa, xb, dinv, tm, 5, lmax / 30.0, lmax, stream=st.ptr)
+        reps = max(10, args.steps // 10)
+        graph.launch()
+        st.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            graph.launch()
+        st.synchronize()
+        extra["sipg_p5_deformed_cheby5_sweep_graph_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 4)
+        del graph, st
         # Cartesian SIPG (the reference's own operator, correct x sweep)
         opc = sfk.Operator(sfk.DG_SIPG, degree, nc, inv_jac=(32, 32, 32))
         tc = timed(lambda: opc.apply(u, v), args.steps, args.warmup)
@@ -210,19 +210,43 @@ def main():
         x1 = (torch.rand(n1, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
               * 2 - 1)
         y1 = torch.empty_like(x1)
+        sched1, _ = o1.tune(repeats=20)
         t1 = timed(lambda: o1.apply(x1, y1), 100, 10)
         extra["q4_16cube_fp64_gdofs"] = round(n1 / t1 / 1e9, 3)
+        extra["q4_16cube_schedule"] = list(sched1)
         log(f"Q4 16^3 continuous: {t1 * 1e6:.1f} us, {n1 / t1 / 1e9:.2f} GDoF/s")
+        # the 100 applies captured in one HIP graph (launch latency removed)
+        st1 = sfk.Stream()
+        torch.cuda.synchronize()
+        o1.apply(x1, y1, stream=st1.ptr)
+        st1.synchronize()
+        with sfk.Graph(st1) as g1:
+            for _ in range(100):
+                o1.apply(x1, y1, stream=st1.ptr)
+        g1.launch()
+        st1.synchronize()
+        t0 = time.perf_counter()
+        g1.launch()
+        st1.synchronize()
+        tg1 = (time.perf_counter() - t0) / 100
+        extra["q4_16cube_fp64_graph_gdofs"] = round(n1 / tg1 / 1e9, 3)
+        del g1, st1
         # c1 scaled to 128^3 cells (HBM regime)
         o1b = sfk.Operator(sfk.CONT_LAPLACE, 4, (128, 128, 128), inv_jac=(128, 128, 128))
         xb1 = torch.randn(o1b.vector_size, dtype=torch.float64, device="cuda")
         yb1 = torch.empty_like(xb1)
+        sched1b, _ = o1b.tune(repeats=3)
         tb1 = timed(lambda: o1b.apply(xb1, yb1), 20, 3)
         extra["q4_128cube_fp64_gdofs"] = round(o1b.vector_size / tb1 / 1e9, 3)
+        extra["q4_128cube_fp64_hbm_gbs"] = round(16 * o1b.vector_size / tb1 / 1e9, 1)
+        extra["q4_128cube_schedule"] = list(sched1b)
+        log(f"Q4 128^3 continuous: {tb1 * 1e3:.3f} ms, {o1b.vector_size / tb1 / 1e9:.2f} GDoF/s")
         del o1b, xb1, yb1
         # c3: mixed-precision CG-Chebyshev on continuous Q6 64^3
         o3 = sfk.Operator(sfk.CONT_LAPLACE, 6, (64, 64, 64), inv_jac=(64, 64, 64))
         s32 = sfk.Operator(sfk.CONT_LAPLACE, 6, (64, 64, 64), inv_jac=(64, 64, 64), precision=sfk.FP32)
+        o3.tune(repeats=3)
+        s32.tune(repeats=3)
         b3 = o3.rhs_constant(1.0)
         x3 = torch.randn(o3.vector_size, dtype=torch.float64, device="cuda")
         y3 = torch.empty_like(x3)
@@ -267,11 +291,14 @@ def main():
                           flags=sfk.FLAG_VARIABLE_COEF, n_rhs=8)
         k5 = np.exp(0.5 * np.random.default_rng(5).standard_normal(o5.n_cell_q))
         o5.set_coefficient(k5)
+        sched5, _ = o5.tune(repeats=5)
+        log(f"var-coef Q8 autotuned schedule: {sched5}")
         x5 = torch.randn(8 * o5.vector_size, dtype=torch.float32, device="cuda")
         y5 = torch.empty_like(x5)
         t5 = timed(lambda: o5.apply(x5, y5), 20, 3)
         t5g = timed(lambda: o5.grad_coefficient(x5, y5), 10, 2)
         extra["varcoef_q8_24cube_8rhs_fp32"] = {"gdofs": round(8 * o5.vector_size / t5 / 1e9, 3),
+                                                "schedule": list(sched5),
                                                 "apply_ms": round(t5 * 1e3, 4),
                                                 "coef_grad_ms": round(t5g * 1e3, 4)}
         log(f"var-coef Q8 24^3 x8 FP32: {8 * o5.vector_size / t5 / 1e9:.2f} GDoF/s")
@@ -298,7 +325,8 @@ def main():
         "data": "synthetic: standard normal input from seed 1, sine-deformed mesh metric generated on the device",
         "config": {"workload": "DG-SIPG Laplacian matvec, degree 5, 32^3 cells deformed by "
                                "x+0.1 sin(pi x)sin(pi y)sin(pi z), per-q metric stored, penalty (p+1)^2/h, FP64",
-                   "n_dofs": N, "degree": degree, "n_cells": list(nc)},
+                   "n_dofs": N, "degree": degree, "n_cells": list(nc),
+                   "schedule": {"cells_per_block": sched[0], "block_order": sched[1]}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
