@@ -3,7 +3,8 @@
 // class): derivative KAT, LSRK time steps, matvec throughput, and the error
 // against the travelling wave.
 //
-//   test_dg_advect [vector_size=300000] [n_tests=10] [degree=-1 (all)]
+//   test_dg_advect [vector_size=300000] [n_tests=10] [degree=-1 (all)] [graph=0]
+// graph=1 replays each time step from a HIP graph of its five stages.
 #include <sys/time.h>
 
 #include <algorithm>
@@ -23,9 +24,10 @@ static double wall()
 }
 
 template <int degree>
-static void run_program(unsigned long vector_size_guess, unsigned n_tests)
+static void run_program(unsigned long vector_size_guess, unsigned n_tests, bool graph)
 {
   EvaluationDGAdvection<3, degree, double> evaluator;
+  evaluator.use_graph = graph;
   const unsigned n_dofs_cell = (degree + 1) * (degree + 1) * (degree + 1);
   const unsigned n_cells_tot = std::max<unsigned>(vector_size_guess / n_dofs_cell, 1U);
   unsigned n_cells[3];
@@ -50,7 +52,7 @@ static void run_program(unsigned long vector_size_guess, unsigned n_tests)
   const std::size_t ops_interpolate = 2 * ((degree + 1) / 2) * 2 + degree + 1 + 2 * ((degree - 1) * (degree + 1) / 2);
   const std::size_t ops_cell = 6 * 3 * ops_interpolate * p1 * p1 + 3 * 4 * ops_interpolate * p1 +
                                3 * 2 * 7 * p1 * p1 + 5 * p1 * p1 * p1;
-  std::cout << "RK degree " << std::setw(2) << degree << "  " << n_cells[0] << " x " << n_cells[1] << " x "
+  std::cout << "RK degree " << std::setw(2) << degree << (graph ? " (graph)" : "") << "  " << n_cells[0] << " x " << n_cells[1] << " x "
             << n_cells[2] << ", MDoFs/s: " << 5. * global_size * n_tests / best / 1e6
             << ", GB/s: " << 5. * global_size * 8 * 3 * n_tests / best * 1e-9
             << " GFLOP/s: " << 5. * evaluator.n_elements() * ops_cell * n_tests / best * 1e-9 << std::endl;
@@ -73,16 +75,17 @@ int main(int argc, char **argv)
   const unsigned long vector_size = argc > 1 ? std::atol(argv[1]) : 300000;
   const unsigned n_tests = argc > 2 ? std::atoi(argv[2]) : 10;
   const int only = argc > 3 ? std::atoi(argv[3]) : -1;
+  const bool graph = argc > 4 && std::atoi(argv[4]) != 0;
   std::cout << sfk_version() << std::endl;
   try {
-    if (only == -1 || only == 1) run_program<1>(vector_size, n_tests);
-    if (only == -1 || only == 2) run_program<2>(vector_size, n_tests);
-    if (only == -1 || only == 3) run_program<3>(vector_size, n_tests);
-    if (only == -1 || only == 4) run_program<4>(vector_size, n_tests);
-    if (only == -1 || only == 5) run_program<5>(vector_size, n_tests);
-    if (only == -1 || only == 6) run_program<6>(vector_size, n_tests);
-    if (only == -1 || only == 7) run_program<7>(vector_size, n_tests);
-    if (only == -1 || only == 8) run_program<8>(vector_size, n_tests);
+    if (only == -1 || only == 1) run_program<1>(vector_size, n_tests, graph);
+    if (only == -1 || only == 2) run_program<2>(vector_size, n_tests, graph);
+    if (only == -1 || only == 3) run_program<3>(vector_size, n_tests, graph);
+    if (only == -1 || only == 4) run_program<4>(vector_size, n_tests, graph);
+    if (only == -1 || only == 5) run_program<5>(vector_size, n_tests, graph);
+    if (only == -1 || only == 6) run_program<6>(vector_size, n_tests, graph);
+    if (only == -1 || only == 7) run_program<7>(vector_size, n_tests, graph);
+    if (only == -1 || only == 8) run_program<8>(vector_size, n_tests, graph);
   } catch (const sfk::Error &e) {
     std::cerr << "error: " << e.what() << std::endl;
     return 2;

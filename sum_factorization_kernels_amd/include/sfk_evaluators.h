@@ -276,6 +276,8 @@ public:
   // solution at the GLL nodes, time step 0.2 h / |a| / p (:160-165)
   void initialize(const unsigned int *n_cells_in)
   {
+    sfk_graph_destroy(graph_); // a captured step refers to the old vectors
+    graph_ = nullptr;
     for (int d = 0; d < 3; ++d) {
       n_cells[d] = n_cells_in[d];
       inv_jac[d] = n_cells[d] / 2.;
@@ -294,21 +296,26 @@ public:
   }
   std::size_t n_elements() const { return std::size_t(n_cells[0]) * n_cells[1] * n_cells[2]; }
 
-  // five-stage, fourth-order low-storage RK (evaluation_dg_advect.h:190-214)
+  // five-stage, fourth-order low-storage RK (evaluation_dg_advect.h:190-214).
+  // With use_graph, the five fused stages are captured once into a HIP
+  // graph on the evaluator's own stream and every later step replays it
+  // (one launch and one synchronisation per step instead of five).
+  bool use_graph = false;
   void do_time_step(double &current_time)
   {
-    const double a21 = 970286171893. / 4311952581923., a32 = 6584761158862. / 12103376702013.,
-                 a43 = 2251764453980. / 15575788980749., a54 = 26877169314380. / 34165994151039.;
-    const double b1 = 1153189308089. / 22510343858157., b2 = 1772645290293. / 4653164025191.,
-                 b3 = -1672844663538. / 4480602732383., b4 = 2114624349019. / 3568978502595.,
-                 b5 = 5198255086312. / 14908931495163.;
-    const double dt = time_step;
-    do_rk_stage(sol_new, sol_new, sol_old, sol_tmp, current_time, a21 * dt, (b1 - a21) * dt);
-    do_rk_stage(sol_old, sol_tmp, sol_new, sol_tmp, current_time, a32 * dt, (b2 - a32) * dt);
-    do_rk_stage(sol_new, sol_tmp, sol_old, sol_tmp, current_time, a43 * dt, (b3 - a43) * dt);
-    do_rk_stage(sol_old, sol_tmp, sol_tmp, sol_new, current_time, a54 * dt, (b4 - a54) * dt);
-    do_rk_stage(sol_tmp, sol_new, sol_new, sol_tmp, current_time, b5 * dt, 0.);
-    current_time += dt;
+    if (use_graph) {
+      if (!graph_ || graph_dt_ != time_step) capture_time_step();
+      sfk::check(sfk_graph_launch(graph_, stream_));
+      sfk::check(sfk_synchronize(stream_));
+    } else {
+      time_step_stages(nullptr, true);
+    }
+    current_time += time_step;
+  }
+  ~EvaluationDGAdvection()
+  {
+    sfk_graph_destroy(graph_);
+    sfk_stream_destroy(stream_);
   }
   void do_matvec() { do_rk_stage<false>(sol_new, sol_new, sol_tmp, sol_tmp, 0., 0., 0.); }
 
@@ -372,7 +379,49 @@ private:
               }
     return v;
   }
+  // the stage sequence of do_time_step (time only enters the reference's
+  // exact solution, not the stages)
+  sfk_status time_step_stages(void *stream, bool sync)
+  {
+    const double a21 = 970286171893. / 4311952581923., a32 = 6584761158862. / 12103376702013.,
+                 a43 = 2251764453980. / 15575788980749., a54 = 26877169314380. / 34165994151039.;
+    const double b1 = 1153189308089. / 22510343858157., b2 = 1772645290293. / 4653164025191.,
+                 b3 = -1672844663538. / 4480602732383., b4 = 2114624349019. / 3568978502595.,
+                 b5 = 5198255086312. / 14908931495163.;
+    const double dt = time_step;
+    sfk_op *op = op_->get();
+    struct Stage {
+      const void *src, *upd;
+      void *np, *tm;
+      double c_np, c_tm;
+    } st[5] = {{sol_new.data(), sol_new.data(), sol_old.data(), sol_tmp.data(), a21 * dt, (b1 - a21) * dt},
+               {sol_old.data(), sol_tmp.data(), sol_new.data(), sol_tmp.data(), a32 * dt, (b2 - a32) * dt},
+               {sol_new.data(), sol_tmp.data(), sol_old.data(), sol_tmp.data(), a43 * dt, (b3 - a43) * dt},
+               {sol_old.data(), sol_tmp.data(), sol_tmp.data(), sol_new.data(), a54 * dt, (b4 - a54) * dt},
+               {sol_tmp.data(), sol_new.data(), sol_new.data(), sol_tmp.data(), b5 * dt, 0.}};
+    for (const Stage &g : st) {
+      const sfk_status s = sfk_rk_stage(op, g.src, g.upd, g.np, g.tm, g.c_np, g.c_tm, stream);
+      if (s != SFK_OK) return s;
+      if (sync) sfk::check(sfk_synchronize(stream));
+    }
+    return SFK_OK;
+  }
+  void capture_time_step()
+  {
+    sfk_graph_destroy(graph_);
+    graph_ = nullptr;
+    if (!stream_) sfk::check(sfk_stream_create(&stream_));
+    sfk::check(sfk_graph_begin(stream_));
+    const sfk_status s = time_step_stages(stream_, false);
+    const sfk_status e = sfk_graph_end(stream_, &graph_); // always end the capture
+    sfk::check(s);
+    sfk::check(e);
+    graph_dt_ = time_step;
+  }
   std::unique_ptr<sfk::Operator> op_;
+  void *stream_ = nullptr;
+  sfk_graph *graph_ = nullptr;
+  double graph_dt_ = 0.;
 };
 
 // ------------------------------------------------------------------ bodies
