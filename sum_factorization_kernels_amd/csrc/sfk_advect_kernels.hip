@@ -24,6 +24,12 @@ constexpr int adv_cells_per_block()
   return 256 / (N * N) > 0 ? 256 / (N * N) : 1;
 }
 constexpr int kAdvMaxThreads = 256;
+template <int N, typename T, int KIND>
+constexpr int adv_waves()
+{
+  constexpr int cb = adv_cells_per_block<N>();
+  return lds_waves_per_simd(cb * (2 * N * N * N + (KIND == 2 ? 0 : 6 * N * N)) * (int)sizeof(T), cb * N * N);
+}
 
 template <int N, typename T>
 struct AdvArgs {
@@ -153,7 +159,8 @@ __device__ __forceinline__ void adv_dir(const AdvArgs<N, T> &a, const T *Acell, 
 }
 
 template <int N, typename T, int KIND, bool VAR, bool STAGE>
-__global__ void __launch_bounds__(kAdvMaxThreads) advect_kernel(const AdvArgs<N, T> a)
+__global__ void __launch_bounds__(kAdvMaxThreads) __attribute__((amdgpu_waves_per_eu(1, adv_waves<N, T, KIND>())))
+advect_kernel(const AdvArgs<N, T> a)
 {
   constexpr int N2 = N * N, N3 = N * N * N;
   constexpr bool ADJ = KIND == 1;

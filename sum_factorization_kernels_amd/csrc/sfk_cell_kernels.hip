@@ -21,7 +21,7 @@
 //   Gz same for z, + B, transposed z value sweep -> A
 //   By transposed y value sweep
 //   Bx transposed x value sweep -> global (DG: fused Chebyshev update;
-//      Cont: colored row stores staged through LDS, or atomic scatter-add)
+//      Cont: colored stores or atomic scatter-add)
 #include "sfk_device.h"
 #include "sfk_kernels.h"
 
@@ -40,6 +40,12 @@ constexpr int threads_per_block()
   return ((cells_per_block<N>() * N * N + 63) / 64) * 64;
 }
 constexpr int kCellMaxThreads = 256;
+template <int N, typename T, bool FACES>
+constexpr int cell_waves()
+{
+  constexpr int cb = cells_per_block<N>();
+  return lds_waves_per_simd(cb * (2 * N * N * N + (FACES ? 12 * N * N : 0)) * (int)sizeof(T), cb * N * N);
+}
 
 template <int N, typename T>
 struct CellArgs {
@@ -123,69 +129,50 @@ __device__ __forceinline__ void grad_phase(const CellArgs<N, T> &a, const T *Ace
   if constexpr (MODE == Mode::DG) sipg_faces<N, DIR, T>(a, q, F, l, r);
 }
 
-// Continuous, colored schedule.  A workgroup owns a "segment": CB
-// consecutive cells of one x row (row y = cy, row z = cz).  Its nodes form
-// N*N rows (j, k) of W = ncells_in_segment*p + 1 contiguous nodes each, which
-// are staged through LDS (S[(j + N*k)*W + x]) so that global loads and
-// stores walk contiguous addresses.  The units (row y, row z, segment) are
-// coloured by the parity of each index: two units of one colour share no
-// node, so a launch writes every node at most once.  The colours run in
-// ascending order; the unit with the smallest colour among those touching a
-// node stores it (Dirichlet nodes: 0), later ones add, which needs neither a
-// zero fill nor atomics.
+// Colored store of one x-line of a continuous cell.  A workgroup owns CB
+// consecutive cells of one x row (a "segment"); the units (row y, row z,
+// segment) are coloured by the parity of each index, so two units of one
+// colour share no node and a launch writes every node at most once.  The
+// colours are launched in ascending order; for every node the unit with the
+// smallest colour among those that touch it stores (Dirichlet nodes: 0) and
+// later ones add, which makes a zero fill and atomics unnecessary.  Inside a
+// segment the first x-plane of cell c+1 (left in LDS by its threads) is
+// added to the last x-plane of cell c.
 template <int N, typename T>
-__device__ __forceinline__ void cont_gather_rows(const CellArgs<N, T> &a, const T *src, T *S, int W, int cx0,
-                                                 int cy, int cz)
+__device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&o)[N], const T *sB, int c, int l,
+                                                   int cx, int cy, int cz, int seg, int cx0, int cx_end, T *dst)
 {
-  constexpr int p = N - 1;
-  const int total = N * N * W, step = blockDim.x;
-  int r = (int)threadIdx.x / W, x = (int)threadIdx.x % W;
-  const int dr = step / W, dx = step % W;
-  for (int idx = threadIdx.x; idx < total; idx += step) {
-    const int X = cx0 * p + x, Y = cy * p + r % N, Z = cz * p + r / N;
-    const bool bd = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
-    S[idx] = bd ? T(0) : src[((long long)Z * a.gy + Y) * a.gx + X];
-    r += dr;
-    x += dx;
-    if (x >= W) {
-      x -= W;
-      ++r;
-    }
-  }
-}
-
-template <int N, typename T>
-__device__ __forceinline__ void cont_scatter_rows(const CellArgs<N, T> &a, const T *S, int W, int cx0, int cy,
-                                                  int cz, int seg, T *dst)
-{
-  constexpr int p = N - 1;
-  const int total = N * N * W, step = blockDim.x;
-  int r = (int)threadIdx.x / W, x = (int)threadIdx.x % W;
-  const int dr = step / W, dx = step % W;
-  for (int idx = threadIdx.x; idx < total; idx += step) {
-    const int j = r % N, k = r / N;
-    // smallest colour of the units touching the node: a range of two rows
-    // (or segments) contains both parities
-    const bool y2 = (j == 0 && cy > 0) || (j == p && cy + 1 < a.ny);
-    const bool z2 = (k == 0 && cz > 0) || (k == p && cz + 1 < a.nz);
-    const bool s2 = (x == 0 && seg > 0) || (x == W - 1 && seg + 1 < a.nseg);
-    const int m = (y2 ? 0 : (cy & 1)) + 2 * (z2 ? 0 : (cz & 1)) + 4 * (s2 ? 0 : (seg & 1));
-    const int X = cx0 * p + x, Y = cy * p + j, Z = cz * p + k;
-    const bool bd = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
-    const long long g = ((long long)Z * a.gy + Y) * a.gx + X;
-    if (m == a.color) dst[g] = bd ? T(0) : S[idx];
-    else if (!bd) dst[g] += S[idx];
-    r += dr;
-    x += dx;
-    if (x >= W) {
-      x -= W;
-      ++r;
-    }
+  constexpr int p = N - 1, N3 = N * N * N;
+  const int j = l % N, k = l / N;
+  if (cx + 1 < cx_end) o[p] += sB[(c + 1) * N3 + l];
+  const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
+  const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;
+  // smallest colour of the units covering rows [ylo,yhi] x [zlo,zhi] x
+  // segments [slo,shi] (each range has one or two entries)
+  auto min_color = [&](int slo, int shi) {
+    const int py = ylo == yhi ? (ylo & 1) : 0, pz = zlo == zhi ? (zlo & 1) : 0, ps = slo == shi ? (slo & 1) : 0;
+    return py + 2 * pz + 4 * ps;
+  };
+  const int m_mid = min_color(seg, seg);
+  const int Y = cy * p + j, Z = cz * p + k;
+  const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
+  const long long g0 = ((long long)Z * a.gy + Y) * a.gx + (long long)cx * p;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i == 0 && cx > cx0) continue; // added into the previous cell's plane
+    int m = m_mid;
+    if (i == 0 && seg > 0) m = min_color(seg - 1, seg);
+    if (i == p && cx + 1 == cx_end && seg + 1 < a.nseg) m = min_color(seg, seg + 1);
+    const int X = cx * p + i;
+    const bool bd = byz || X == 0 || X == a.gx - 1;
+    if (m == a.color) dst[g0 + i] = bd ? T(0) : o[i];
+    else if (!bd) dst[g0 + i] += o[i];
   }
 }
 
 template <int N, typename T, Mode MODE, bool CHEBY, bool REFX, bool COEF>
-__global__ void __launch_bounds__(kCellMaxThreads) cell_kernel(const CellArgs<N, T> a)
+__global__ void __launch_bounds__(kCellMaxThreads) __attribute__((amdgpu_waves_per_eu(1, cell_waves<N, T, MODE == Mode::DG>())))
+cell_kernel(const CellArgs<N, T> a)
 {
   constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
   constexpr bool FACES = MODE == Mode::DG;
@@ -224,28 +211,20 @@ __global__ void __launch_bounds__(kCellMaxThreads) cell_kernel(const CellArgs<N,
     }
   }
   const long long cell = cell0 + c;
-  const int W = (cx_end - cx0) * p + 1; // continuous colored: nodes per staged row
   T *A = sA + (act ? c : 0) * N3;
   T *B = sB + (act ? c : 0) * N3;
   T *F = sF + (FACES && act ? c * 12 * N2 : 0);
 
-  for (int rr = 0; rr < a.nrhs; ++rr) {
+  // right-hand side rr = blockIdx.y (a loop over the right-hand sides here
+  // would keep every 1D table live across it and spill scalar registers)
+  {
+    const long long rr = blockIdx.y;
     const T *src = a.src + rr * a.vstride;
     T *dst = a.dst + rr * a.vstride;
     // ---------------------------------------------------------------- L
-    if constexpr (MODE == Mode::Cont) {
-      if (a.color >= 0) {
-        cont_gather_rows<N, T>(a, src, sB, W, cx0, ccy, ccz);
-        __syncthreads();
-      }
-    }
     if (act) {
       T u[N], v[N];
-      if (MODE == Mode::Cont && a.color >= 0) {
-        const int i = l % N, j = l / N;
-#pragma unroll
-        for (int k = 0; k < N; ++k) u[k] = sB[(j + N * k) * W + c * p + i];
-      } else if constexpr (MODE == Mode::Cont) {
+      if constexpr (MODE == Mode::Cont) {
         const int X = ccx * p + l % N, Y = ccy * p + l / N;
         const bool bxy = X == 0 || Y == 0 || X == a.gx - 1 || Y == a.gy - 1;
         const long long g0 = (long long)ccz * p * a.gy * a.gx + (long long)Y * a.gx + X;
@@ -371,19 +350,15 @@ __global__ void __launch_bounds__(kCellMaxThreads) cell_kernel(const CellArgs<N,
         eo_apply<N, 1>(tb.Sf, u, o); // evaluation_dg_laplacian.h:481-483
       else
         eo_apply<N, 1>(tb.Sb, u, o);
+      if constexpr (MODE == Mode::Cont) {
+        // x-merge: the first x-plane of cell c+1 is the last one of cell c
+        if (a.color >= 0) B[l] = o[0];
+      }
     }
     if constexpr (MODE == Mode::Cont) {
       if (a.color >= 0) {
-        // stage the segment's rows: cell c owns nodes c*p .. c*p+p of row l;
-        // the shared node c*p of cells c-1 and c is summed in a second pass
-        if (act)
-#pragma unroll
-          for (int i = 0; i < N; ++i)
-            if (i > 0 || c == 0) sB[l * W + c * p + i] = o[i];
         __syncthreads();
-        if (act && c > 0) sB[l * W + c * p] += o[0];
-        __syncthreads();
-        cont_scatter_rows<N, T>(a, sB, W, cx0, ccy, ccz, seg, dst);
+        if (act) cont_store_colored<N, T>(a, o, sB, c, l, ccx, ccy, ccz, seg, cx0, cx_end, dst);
       } else if (act) {
         const int Y = ccy * p + l % N, Z = ccz * p + l / N;
         const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
@@ -415,7 +390,6 @@ __global__ void __launch_bounds__(kCellMaxThreads) cell_kernel(const CellArgs<N,
         for (int i = 0; i < N; ++i) dst[g0 + i] = o[i];
       }
     }
-    if (rr + 1 < a.nrhs) __syncthreads();
   }
 }
 
@@ -704,7 +678,7 @@ static hipError_t launch_cell_n(const CellLaunch &L, hipStream_t stream)
   constexpr int N3 = N * N * N;
   const size_t lds = size_t(CB) * (2 * N3 + (L.mode == 2 ? 12 * N * N : 0)) * sizeof(T);
   const unsigned blocks = (unsigned)((L.ncells + CB - 1) / CB);
-  const dim3 grid(blocks), block(((CB * N * N + 63) / 64) * 64);
+  const dim3 grid(blocks, L.nrhs), block(((CB * N * N + 63) / 64) * 64);
   auto launch = [&](auto kern, dim3 g) {
     const hipError_t e = allow_dynamic_lds((const void *)kern, lds);
     if (e != hipSuccess) return e;
@@ -733,7 +707,7 @@ static hipError_t launch_cell_n(const CellLaunch &L, hipStream_t stream)
       a.nrz = (a.nz - pz + 1) / 2;
       const long long nb = (long long)a.nss * a.nry * a.nrz;
       if (nb == 0) continue;
-      const hipError_t e = launch(kern, dim3((unsigned)nb));
+      const hipError_t e = launch(kern, dim3((unsigned)nb, L.nrhs));
       if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -170,6 +170,20 @@ __device__ __forceinline__ unsigned logical_block(unsigned b, unsigned nb, int o
   return x * q + (x < r ? x : r) + k;
 }
 
+// Waves per SIMD that a kernel's LDS allows at its default cells per
+// workgroup (gfx950: 160 KiB of LDS per CU, 4 SIMDs, at most 8 waves per
+// SIMD).  The operator kernels size their LDS at run time (dynamic LDS), so
+// the compiler cannot see that LDS caps their occupancy; passing this bound
+// as amdgpu_waves_per_eu(1, W) lets it spend up to 512/W registers per lane
+// instead of squeezing them for waves that can never be resident.
+constexpr int lds_waves_per_simd(int lds_bytes_per_block, int threads_per_block)
+{
+  const int blocks = lds_bytes_per_block > 0 ? 163840 / lds_bytes_per_block : 32;
+  const int waves = blocks * ((threads_per_block + 63) / 64);
+  const int w = (waves + 3) / 4;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
 // read-once streams (stored metric): non-temporal loads keep them from
 // evicting the re-read neighbour data out of L2
 template <typename T>

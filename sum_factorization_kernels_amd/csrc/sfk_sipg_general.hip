@@ -42,6 +42,12 @@ constexpr int gen_cells_per_block()
   return N <= 3 ? 256 / (N * N) : (N == 4 ? 12 : (N == 5 ? 8 : (N == 6 ? 5 : (N == 7 ? 3 : (N == 8 ? 2 : 1)))));
 }
 constexpr int kGenMaxThreads = 256;
+template <int N, typename T, bool DIAG>
+constexpr int gen_waves()
+{
+  constexpr int cb = gen_cells_per_block<N>();
+  return lds_waves_per_simd(cb * (4 * N * N * N + (DIAG ? 0 : 12 * N * N)) * (int)sizeof(T), cb * N * N);
+}
 
 template <int N, typename T>
 struct GenArgs {
@@ -140,7 +146,8 @@ __device__ __forceinline__ void inject_grad(const Tab<N, T> &tb, T *Tx, T *Ty, T
 }
 
 template <int N, typename T, bool CHEBY, bool DIAG>
-__global__ void __launch_bounds__(kGenMaxThreads) sipg_general_kernel(const GenArgs<N, T> a)
+__global__ void __launch_bounds__(kGenMaxThreads) __attribute__((amdgpu_waves_per_eu(1, gen_waves<N, T, DIAG>())))
+sipg_general_kernel(const GenArgs<N, T> a)
 {
   constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
   const int CB = a.cb;
