@@ -33,7 +33,7 @@ constexpr int adv_waves()
 
 template <int N, typename T>
 struct AdvArgs {
-  Tab<N, T> tab;
+  const DevTab<N, T> *dt; // tables in device memory (read through ctab)
   const T *src, *upd, *aux;
   T *dst, *tm, *dst2;
   T c_np, c_tm;
@@ -95,8 +95,8 @@ __device__ __forceinline__ void load_face_layers(const AdvArgs<N, T> &a, const T
   }
 }
 
-template <int N, typename T>
-__device__ __forceinline__ void face_sweep(const Eo<N, T> &M, T *F, int narr, int l, bool lo)
+template <int N, typename T, typename E>
+__device__ __forceinline__ void face_sweep_t(const E &M, T *F, int narr, int l, bool lo)
 {
   constexpr int N2 = N * N;
   for (int it = l; it < narr * N; it += N2) {
@@ -116,7 +116,7 @@ template <int N, int DIR, typename T, bool VAR, bool ADJ>
 __device__ __forceinline__ void adv_dir(const AdvArgs<N, T> &a, const T *Acell, const T *F, int l, long long cell,
                                         T (&r)[N])
 {
-  const Tab<N, T> &tb = a.tab;
+  const auto &tb = ctab(a.dt);
   T q[N];
   lds_read_line<N, N, DIR>(Acell, l, q);
   const T c = a.ij[DIR] * a.jxw;
@@ -169,7 +169,6 @@ advect_kernel(const AdvArgs<N, T> a)
   T *sA = dyn_lds<T>();
   T *sB = sA + CB * N3;
   T *sF = sB + CB * N3;
-  const Tab<N, T> &tb = a.tab;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
   const long long cell0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB, cell = cell0 + c;
   const bool act = c < CB && cell < a.ncells;
@@ -183,38 +182,38 @@ advect_kernel(const AdvArgs<N, T> a)
       T u[N], v[N];
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(tb.Ib, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Ib, u, v);
       lds_write_line<N, N, 2>(A, l, v);
     }
     __syncthreads();
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(tb.Ib, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Ib, u, v);
       lds_write_line<N, N, 0>(A, l, v);
     }
     __syncthreads();
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(tb.Ib, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Ib, u, v);
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] /= a.jxw * line_weight<N, 1>(tb.w, l, i);
-      eo_apply<N, 1>(tb.If, v, u);
+      for (int i = 0; i < N; ++i) v[i] /= a.jxw * line_weight<N, 1>(ctab(a.dt).w, l, i);
+      eo_apply<N, 1>(ctab(a.dt).If, v, u);
       lds_write_line<N, N, 1>(A, l, u);
     }
     __syncthreads();
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(tb.If, u, v);
+      eo_apply<N, 1>(ctab(a.dt).If, u, v);
       lds_write_line<N, N, 0>(A, l, v);
     }
     __syncthreads();
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 2>(A, l, u);
-      eo_apply<N, 1>(tb.If, u, v);
+      eo_apply<N, 1>(ctab(a.dt).If, u, v);
 #pragma unroll
       for (int k = 0; k < N; ++k) a.dst[cell * N3 + k * N2 + l] = v[k];
     }
@@ -226,11 +225,11 @@ advect_kernel(const AdvArgs<N, T> a)
       T u[N], v[N];
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 2>(A, l, v);
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.aux[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 2>(B, l, v);
     }
     for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
@@ -250,23 +249,23 @@ advect_kernel(const AdvArgs<N, T> a)
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 0>(A, l, v);
       lds_read_line<N, N, 0>(B, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 0>(B, l, v);
-      face_sweep<N, T>(tb.Sf, F, 6, l, true);
+      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, true);
     }
     __syncthreads();
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 1>(A, l, v);
       lds_read_line<N, N, 1>(B, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 1>(B, l, v);
-      face_sweep<N, T>(tb.Sf, F, 6, l, false);
+      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, false);
     }
     __syncthreads();
     if (act) {
@@ -275,25 +274,25 @@ advect_kernel(const AdvArgs<N, T> a)
         T uq[N], mq[N], g[N];
         lds_read_line<N, N, DIR>(A, l, uq);
         lds_read_line<N, N, DIR>(B, l, mq);
-        eo_apply<N, -1>(tb.Df, mq, g);
+        eo_apply<N, -1>(ctab(a.dt).Df, mq, g);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           const int dof = line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
           a.dst[(cell * 3 + DIR) * N3 + dof] =
-              g[i] * a.ij[DIR] * uq[i] * (a.jxw * line_weight<N, DIR>(tb.w, l, i));
+              g[i] * a.ij[DIR] * uq[i] * (a.jxw * line_weight<N, DIR>(ctab(a.dt).w, l, i));
         }
         T um = 0, mm = 0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          um += tb.fv1[i] * uq[i];
-          mm += tb.fv1[i] * mq[i];
+          um += ctab(a.dt).fv1[i] * uq[i];
+          mm += ctab(a.dt).fv1[i] * mq[i];
         }
         const T up = F[DIR * N2 + l], mp = F[(3 + DIR) * N2 + l];
         const T b = a.face_vel[((long long)DIR * a.ncells + cell) * N2 + l];
         const T sg = b > T(0) ? T(1) : (b < T(0) ? T(-1) : T(0));
         const T uh = T(0.5) * (um + up) + T(0.5) * sg * (um - up);
         a.dst2[((long long)DIR * a.ncells + cell) * N2 + l] =
-            -(mm - mp) * uh * (tb.w[l % N] * tb.w[l / N] * a.jxwf[DIR]);
+            -(mm - mp) * uh * (ctab(a.dt).w[l % N] * ctab(a.dt).w[l / N] * a.jxwf[DIR]);
       };
       do_dir(std::integral_constant<int, 0>());
       do_dir(std::integral_constant<int, 1>());
@@ -306,7 +305,7 @@ advect_kernel(const AdvArgs<N, T> a)
       T u[N], v[N];
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 2>(A, l, v);
     }
     load_face_layers<N, T>(a, a.src, sF, cell0, CB, 0, 6);
@@ -314,17 +313,17 @@ advect_kernel(const AdvArgs<N, T> a)
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 0>(A, l, v);
-      face_sweep<N, T>(tb.Sf, F, 6, l, true);
+      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, true);
     }
     __syncthreads();
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 1>(A, l, v);
-      face_sweep<N, T>(tb.Sf, F, 6, l, false);
+      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, false);
     }
     __syncthreads();
     if (act) {
@@ -351,10 +350,10 @@ advect_kernel(const AdvArgs<N, T> a)
       if constexpr (!ADJ) {
         // inverse mass matrix by collocation, evaluation_dg_advect.h:619-624
 #pragma unroll
-        for (int i = 0; i < N; ++i) r[i] /= a.jxw * line_weight<N, 2>(tb.w, l, i);
-        eo_apply<N, 1>(tb.If, r, o);
+        for (int i = 0; i < N; ++i) r[i] /= a.jxw * line_weight<N, 2>(ctab(a.dt).w, l, i);
+        eo_apply<N, 1>(ctab(a.dt).If, r, o);
       } else {
-        eo_apply<N, 1>(tb.Sb, r, o);
+        eo_apply<N, 1>(ctab(a.dt).Sb, r, o);
       }
       lds_write_line<N, N, 2>(A, l, o);
     }
@@ -362,16 +361,16 @@ advect_kernel(const AdvArgs<N, T> a)
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, v);
-      else eo_apply<N, 1>(tb.Sb, u, v);
+      if constexpr (!ADJ) eo_apply<N, 1>(ctab(a.dt).If, u, v);
+      else eo_apply<N, 1>(ctab(a.dt).Sb, u, v);
       lds_write_line<N, N, 1>(A, l, v);
     }
     __syncthreads();
     if (act) {
       T u[N], o[N];
       lds_read_line<N, N, 0>(A, l, u);
-      if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, o);
-      else eo_apply<N, 1>(tb.Sb, u, o);
+      if constexpr (!ADJ) eo_apply<N, 1>(ctab(a.dt).If, u, o);
+      else eo_apply<N, 1>(ctab(a.dt).Sb, u, o);
       const long long g0 = cell * N3 + l * N;
       if constexpr (STAGE) {
         // evaluation_dg_advect.h:651-667
@@ -389,32 +388,12 @@ advect_kernel(const AdvArgs<N, T> a)
   }
 }
 
-template <int N, typename T>
-static Tab<N, T> adv_tab(const Tables1D &t)
-{
-  Tab<N, T> tb{};
-  tb.Sf = make_eo<N, T>(t.S, true);
-  tb.Sb = make_eo<N, T>(t.S, false);
-  tb.Df = make_eo<N, T>(t.D, true);
-  tb.Db = make_eo<N, T>(t.D, false);
-  tb.If = make_eo<N, T>(t.Sinv, true);
-  tb.Ib = make_eo<N, T>(t.Sinv, false);
-  for (int i = 0; i < N; ++i) {
-    tb.fv0[i] = T(t.fv0[i]);
-    tb.fv1[i] = T(t.fv1[i]);
-    tb.fd0[i] = T(t.fd0[i]);
-    tb.fd1[i] = T(t.fd1[i]);
-    tb.w[i] = T(t.weights[i]);
-  }
-  tb.hd = T(t.hermite_d);
-  return tb;
-}
 
 template <int N, typename T>
 static hipError_t launch_advect_n(const AdvectLaunch &L, hipStream_t s)
 {
   AdvArgs<N, T> a{};
-  a.tab = adv_tab<N, T>(*L.tables);
+  a.dt = (const DevTab<N, T> *)L.dtab;
   a.src = (const T *)L.src;
   a.upd = (const T *)L.upd;
   a.aux = (const T *)L.aux;

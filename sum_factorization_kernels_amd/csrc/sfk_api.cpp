@@ -29,6 +29,12 @@ using namespace sfk;
     }                                                                                       \
   } while (0)
 
+#define SFK_ENSURE_TABLES(op)                             \
+  do {                                                    \
+    const sfk_status st_ = sfk::ensure_device_tables(op); \
+    if (st_ != SFK_OK) return st_;                        \
+  } while (0)
+
 #define SFK_REQUIRE(cond, code, msg) \
   do {                               \
     if (!(cond)) {                   \
@@ -65,6 +71,19 @@ sfk_status upload(sfk_op *op, void **dev, const double *host, size_t count, size
   return SFK_OK;
 }
 
+// the operator's 1D tables in device memory, uploaded on the first launch
+sfk_status ensure_device_tables(sfk_op *op)
+{
+  if (op->dtab) return SFK_OK;
+  const hipError_t e = upload_device_tables(op->tab, op->d.precision == SFK_FP32, &op->dtab);
+  if (e != hipSuccess) {
+    op->dtab = nullptr;
+    set_error(std::string("device tables: ") + hipGetErrorString(e));
+    return SFK_ERR_HIP;
+  }
+  return SFK_OK;
+}
+
 sfk_status ensure_scratch(sfk_op *op, size_t bytes)
 {
   if (op->scratch_bytes >= bytes) return SFK_OK;
@@ -92,6 +111,7 @@ void fill_cell_launch(const sfk_op *op, CellLaunch &L)
   L.nrhs = op->d.n_rhs;
   L.vstride = op->nvec;
   L.coef = op->coef;
+  L.dtab = op->dtab;
   L.cont_scatter = op->cont_scatter;
   L.cb = op->sched_cb;
   L.order = op->sched_order;
@@ -170,7 +190,7 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
 sfk_status sfk_op_destroy(sfk_op *op)
 {
   if (!op) return SFK_OK;
-  for (void *p : {op->cell_metric, op->face_metric, op->coef, op->cell_vel, op->face_vel, op->scratch})
+  for (void *p : {op->cell_metric, op->face_metric, op->coef, op->cell_vel, op->face_vel, op->scratch, op->dtab})
     if (p) (void)hipFree(p);
   delete op;
   return SFK_OK;
@@ -289,6 +309,7 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream)
   case SFK_CONT_LAPLACE: {
     SFK_REQUIRE(!(d.flags & SFK_FLAG_VARIABLE_COEF) || op->coef, SFK_ERR_NOT_READY, "coefficient not set");
     CellLaunch L;
+    SFK_ENSURE_TABLES(op);
     fill_cell_launch(op, L);
     L.mode = d.kind == SFK_CELL_LAPLACE ? 0 : 1;
     L.src = src;
@@ -299,6 +320,7 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream)
   case SFK_DG_SIPG: {
     if (d.geometry == SFK_GEOM_CARTESIAN) {
       CellLaunch L;
+      SFK_ENSURE_TABLES(op);
       fill_cell_launch(op, L);
       L.mode = 2;
       L.refx = (d.flags & SFK_FLAG_REF_X_SWEEP) != 0;
@@ -308,6 +330,7 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream)
     } else {
       SFK_REQUIRE(op->cell_metric && op->face_metric, SFK_ERR_NOT_READY, "metric not set");
       SipgLaunch L;
+      SFK_ENSURE_TABLES(op);
       fill_sipg_launch(op, L);
       L.src = src;
       L.dst = dst;
@@ -318,6 +341,7 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream)
   case SFK_DG_ADVECT: {
     SFK_REQUIRE(!(d.flags & SFK_FLAG_VARIABLE_VELOCITY) || op->cell_vel, SFK_ERR_NOT_READY, "velocity not set");
     AdvectLaunch L;
+    SFK_ENSURE_TABLES(op);
     fill_advect_launch(op, L);
     L.kind = 0;
     L.src = src;
@@ -345,6 +369,7 @@ sfk_status sfk_apply_transpose(sfk_op *op, const void *src, void *dst, void *str
   if (st != SFK_OK) return st;
   // A^T w = L^T M^{-1} w: cell-local mass inverse, then the transposed weak form
   AdvectLaunch L;
+  SFK_ENSURE_TABLES(op);
   fill_advect_launch(op, L);
   L.kind = 2;
   L.src = src;
@@ -368,6 +393,7 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
   if (d.kind == SFK_DG_SIPG) {
     if (d.geometry == SFK_GEOM_CARTESIAN) {
       CellLaunch L;
+      SFK_ENSURE_TABLES(op);
       fill_cell_launch(op, L);
       L.mode = 2;
       L.cheby = true;
@@ -383,6 +409,7 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
     } else {
       SFK_REQUIRE(op->cell_metric && op->face_metric, SFK_ERR_NOT_READY, "metric not set");
       SipgLaunch L;
+      SFK_ENSURE_TABLES(op);
       fill_sipg_launch(op, L);
       L.cheby = true;
       L.src = sol_old;
@@ -419,6 +446,7 @@ sfk_status sfk_rk_stage(sfk_op *op, const void *src, const void *upd, void *vec_
               "sfk_rk_stage: the stage input must not be overwritten");
   SFK_REQUIRE(!(op->d.flags & SFK_FLAG_VARIABLE_VELOCITY) || op->cell_vel, SFK_ERR_NOT_READY, "velocity not set");
   AdvectLaunch L;
+  SFK_ENSURE_TABLES(op);
   fill_advect_launch(op, L);
   L.kind = 0;
   L.stage = true;
@@ -441,6 +469,7 @@ sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream)
   if (d.kind == SFK_CELL_LAPLACE || d.kind == SFK_CONT_LAPLACE) {
     SFK_REQUIRE(!(d.flags & SFK_FLAG_VARIABLE_COEF) || op->coef, SFK_ERR_NOT_READY, "coefficient not set");
     CellLaunch L;
+    SFK_ENSURE_TABLES(op);
     fill_cell_launch(op, L);
     L.mode = d.kind == SFK_CELL_LAPLACE ? 0 : 1;
     L.dst = diag;
@@ -453,6 +482,7 @@ sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream)
                 "DG-SIPG diagonal: per-q geometry (use the identity-mapped metric for Cartesian meshes)");
     SFK_REQUIRE(op->cell_metric && op->face_metric, SFK_ERR_NOT_READY, "metric not set");
     SipgLaunch L;
+    SFK_ENSURE_TABLES(op);
     fill_sipg_launch(op, L);
     L.diagonal = true;
     L.dst = diag;
@@ -469,6 +499,7 @@ sfk_status sfk_op_rhs_constant(sfk_op *op, double f, void *b, void *stream)
   SFK_REQUIRE(op->d.kind == SFK_CONT_LAPLACE, SFK_ERR_UNSUPPORTED, "sfk_op_rhs_constant: continuous Laplacian only");
   hipStream_t s = (hipStream_t)stream;
   CellLaunch L;
+  SFK_ENSURE_TABLES(op);
   fill_cell_launch(op, L);
   L.mode = 1;
   L.dst = b;
@@ -491,6 +522,7 @@ sfk_status sfk_grad_velocity(sfk_op *op, const void *u, const void *lambda, void
   if (st != SFK_OK) return st;
   // mu = M^{-1} lambda, then dJ/da = mu^T dL/da u
   AdvectLaunch L;
+  SFK_ENSURE_TABLES(op);
   fill_advect_launch(op, L);
   L.kind = 2;
   L.src = lambda;
@@ -510,8 +542,10 @@ sfk_status sfk_grad_coefficient(sfk_op *op, const void *u, const void *lambda, v
   SFK_REQUIRE(op && u && lambda && grad_k, SFK_ERR_INVALID_ARG, "sfk_grad_coefficient: null argument");
   SFK_REQUIRE((op->d.kind == SFK_CONT_LAPLACE) && (op->d.flags & SFK_FLAG_VARIABLE_COEF), SFK_ERR_UNSUPPORTED,
               "sfk_grad_coefficient: continuous Laplacian with SFK_FLAG_VARIABLE_COEF only");
+  SFK_ENSURE_TABLES(op);
   CoefGradLaunch L;
   L.tables = &op->tab;
+  L.dtab = op->dtab;
   L.fp32 = op->d.precision == SFK_FP32;
   L.u = u;
   L.lambda = lambda;

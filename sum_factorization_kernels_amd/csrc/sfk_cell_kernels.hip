@@ -49,7 +49,7 @@ constexpr int cell_waves()
 
 template <int N, typename T>
 struct CellArgs {
-  Tab<N, T> tab;
+  const DevTab<N, T> *dt; // tables in device memory (read through ctab)
   const T *src;
   T *dst;
   const T *rhs, *diag; // Chebyshev
@@ -82,7 +82,7 @@ __device__ __forceinline__ void sipg_faces(const CellArgs<N, T> &a, const T (&q)
                                            T (&r)[N])
 {
   constexpr int N2 = N * N;
-  const Tab<N, T> &tb = a.tab;
+  const auto &tb = ctab(a.dt);
   T v0 = 0, v1 = 0, d0 = 0, d1 = 0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -111,7 +111,7 @@ template <int N, int DIR, typename T, Mode MODE, bool COEF>
 __device__ __forceinline__ void grad_phase(const CellArgs<N, T> &a, const T *Acell, const T *F, int l,
                                            long long cell, T (&r)[N])
 {
-  const Tab<N, T> &tb = a.tab;
+  const auto &tb = ctab(a.dt);
   T q[N], g[N];
   lds_read_line<N, N, DIR>(Acell, l, q);
   eo_apply<N, -1>(tb.Df, q, g);
@@ -182,7 +182,6 @@ cell_kernel(const CellArgs<N, T> a)
   T *sB = sA + CB * N3;
   T *sF = sB + CB * N3;
 
-  const Tab<N, T> &tb = a.tab;
   const int t = threadIdx.x;
   const int c = t / N2, l = t % N2;
   const unsigned blk = logical_block(blockIdx.x, gridDim.x, a.order);
@@ -238,7 +237,7 @@ cell_kernel(const CellArgs<N, T> a)
 #pragma unroll
         for (int k = 0; k < N; ++k) u[k] = src[cell * N3 + k * N2 + l];
       }
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 2>(A, l, v);
     }
     if constexpr (FACES) {
@@ -263,7 +262,7 @@ cell_kernel(const CellArgs<N, T> a)
           oi = face_dof<N, 2>(li, fa);
         }
         const T uf = src[nb + of], ui = src[nb + oi];
-        const T w0 = s ? -tb.hd : tb.hd;
+        const T w0 = s ? -ctab(a.dt).hd : ctab(a.dt).hd;
         sF[(cc * 12 + f) * N2 + fa] = uf;
         sF[(cc * 12 + 6 + f) * N2 + fa] = w0 * (ui - uf);
       }
@@ -273,14 +272,14 @@ cell_kernel(const CellArgs<N, T> a)
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 0>(A, l, v);
       if constexpr (FACES) {
         for (int it = l; it < 12 * N; it += N2) {
           T *fl = F + (it / N) * N2 + (it % N) * N; // lo-direction line
 #pragma unroll
           for (int i = 0; i < N; ++i) u[i] = fl[i];
-          eo_apply<N, 1>(tb.Sf, u, v);
+          eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
 #pragma unroll
           for (int i = 0; i < N; ++i) fl[i] = v[i];
         }
@@ -291,14 +290,14 @@ cell_kernel(const CellArgs<N, T> a)
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 1>(A, l, v);
       if constexpr (FACES) {
         for (int it = l; it < 12 * N; it += N2) {
           T *fl = F + (it / N) * N2 + (it % N); // hi-direction line
 #pragma unroll
           for (int i = 0; i < N; ++i) u[i] = fl[i * N];
-          eo_apply<N, 1>(tb.Sf, u, v);
+          eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
 #pragma unroll
           for (int i = 0; i < N; ++i) fl[i * N] = v[i];
         }
@@ -329,7 +328,7 @@ cell_kernel(const CellArgs<N, T> a)
       lds_read_line<N, N, 2>(B, l, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) r[i] += b[i];
-      eo_apply<N, 1>(tb.Sb, r, o);
+      eo_apply<N, 1>(ctab(a.dt).Sb, r, o);
       lds_write_line<N, N, 2>(A, l, o);
     }
     __syncthreads();
@@ -337,7 +336,7 @@ cell_kernel(const CellArgs<N, T> a)
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(tb.Sb, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sb, u, v);
       lds_write_line<N, N, 1>(A, l, v);
     }
     __syncthreads();
@@ -347,9 +346,9 @@ cell_kernel(const CellArgs<N, T> a)
       T u[N];
       lds_read_line<N, N, 0>(A, l, u);
       if constexpr (REFX)
-        eo_apply<N, 1>(tb.Sf, u, o); // evaluation_dg_laplacian.h:481-483
+        eo_apply<N, 1>(ctab(a.dt).Sf, u, o); // evaluation_dg_laplacian.h:481-483
       else
-        eo_apply<N, 1>(tb.Sb, u, o);
+        eo_apply<N, 1>(ctab(a.dt).Sb, u, o);
       if constexpr (MODE == Mode::Cont) {
         // x-merge: the first x-plane of cell c+1 is the last one of cell c
         if (a.color >= 0) B[l] = o[0];
@@ -486,7 +485,7 @@ __global__ void __launch_bounds__(threads_per_block<N>()) diag_kernel(const Diag
 // at every cell quadrature point.
 template <int N, typename T>
 struct CoefGradArgs {
-  Tab<N, T> tab;
+  const DevTab<N, T> *dt;
   const T *u, *lam;
   T *grad;
   long long ncells;
@@ -504,7 +503,6 @@ __global__ void __launch_bounds__(threads_per_block<N>()) coef_grad_kernel(const
   __shared__ T sU[CB * N3];
   __shared__ T sL[CB * N3];
   __shared__ T sG[CB * N3];
-  const Tab<N, T> &tb = a.tab;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
   const long long cell = (long long)blockIdx.x * CB + c;
   const bool act = c < CB && cell < a.ncells;
@@ -525,8 +523,8 @@ __global__ void __launch_bounds__(threads_per_block<N>()) coef_grad_kernel(const
         u[k] = bd ? T(0) : a.u[g];
         w[k] = bd ? T(0) : a.lam[g];
       }
-      eo_apply<N, 1>(tb.Sf, u, v);
-      eo_apply<N, 1>(tb.Sf, w, z);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, w, z);
       lds_write_line<N, N, 2>(U, l, v);
       lds_write_line<N, N, 2>(Lm, l, z);
     }
@@ -534,20 +532,20 @@ __global__ void __launch_bounds__(threads_per_block<N>()) coef_grad_kernel(const
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 0>(U, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 0>(U, l, v);
       lds_read_line<N, N, 0>(Lm, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 0>(Lm, l, v);
     }
     __syncthreads();
     if (act) {
       T u[N], v[N];
       lds_read_line<N, N, 1>(U, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 1>(U, l, v);
       lds_read_line<N, N, 1>(Lm, l, u);
-      eo_apply<N, 1>(tb.Sf, u, v);
+      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
       lds_write_line<N, N, 1>(Lm, l, v);
     }
     __syncthreads();
@@ -556,8 +554,8 @@ __global__ void __launch_bounds__(threads_per_block<N>()) coef_grad_kernel(const
       T u[N], w[N], gu[N], gw[N], acc[N];
       lds_read_line<N, N, DIR>(U, l, u);
       lds_read_line<N, N, DIR>(Lm, l, w);
-      eo_apply<N, -1>(tb.Df, u, gu);
-      eo_apply<N, -1>(tb.Df, w, gw);
+      eo_apply<N, -1>(ctab(a.dt).Df, u, gu);
+      eo_apply<N, -1>(ctab(a.dt).Df, w, gw);
       if (first) {
 #pragma unroll
         for (int i = 0; i < N; ++i) acc[i] = T(0);
@@ -565,7 +563,7 @@ __global__ void __launch_bounds__(threads_per_block<N>()) coef_grad_kernel(const
         lds_read_line<N, N, DIR>(G, l, acc);
       }
 #pragma unroll
-      for (int i = 0; i < N; ++i) acc[i] += a.cdiag[DIR] * line_weight<N, DIR>(tb.w, l, i) * gu[i] * gw[i];
+      for (int i = 0; i < N; ++i) acc[i] += a.cdiag[DIR] * line_weight<N, DIR>(ctab(a.dt).w, l, i) * gu[i] * gw[i];
       lds_write_line<N, N, DIR>(G, l, acc);
     };
     if (act) dir_term(std::integral_constant<int, 0>(), r == 0);
@@ -583,14 +581,12 @@ __global__ void __launch_bounds__(threads_per_block<N>()) coef_grad_kernel(const
   }
 }
 
-template <int N, typename T>
-static Tab<N, T> make_tab(const Tables1D &t);
 
 template <int N, typename T>
 static hipError_t launch_coef_grad_n(const CoefGradLaunch &L, hipStream_t s)
 {
   CoefGradArgs<N, T> a{};
-  a.tab = make_tab<N, T>(*L.tables);
+  a.dt = (const DevTab<N, T> *)L.dtab;
   a.u = (const T *)L.u;
   a.lam = (const T *)L.lambda;
   a.grad = (T *)L.grad;
@@ -612,26 +608,6 @@ static hipError_t launch_coef_grad_n(const CoefGradLaunch &L, hipStream_t s)
 }
 
 // ---------------------------------------------------------------- launch
-template <int N, typename T>
-static Tab<N, T> make_tab(const Tables1D &t)
-{
-  Tab<N, T> tb{};
-  tb.Sf = make_eo<N, T>(t.S, true);
-  tb.Sb = make_eo<N, T>(t.S, false);
-  tb.Df = make_eo<N, T>(t.D, true);
-  tb.Db = make_eo<N, T>(t.D, false);
-  tb.If = make_eo<N, T>(t.Sinv, true);
-  tb.Ib = make_eo<N, T>(t.Sinv, false);
-  for (int i = 0; i < N; ++i) {
-    tb.fv0[i] = T(t.fv0[i]);
-    tb.fv1[i] = T(t.fv1[i]);
-    tb.fd0[i] = T(t.fd0[i]);
-    tb.fd1[i] = T(t.fd1[i]);
-    tb.w[i] = T(t.weights[i]);
-  }
-  tb.hd = T(t.hermite_d);
-  return tb;
-}
 
 // continuous scatter schedule: colored (no atomics, no zero fill) unless the
 // caller asks for the atomic variant or, by default, each of the 8 colour
@@ -646,7 +622,7 @@ template <int N, typename T>
 static hipError_t launch_cell_n(const CellLaunch &L, hipStream_t stream)
 {
   CellArgs<N, T> a{};
-  a.tab = make_tab<N, T>(*L.tables);
+  a.dt = (const DevTab<N, T> *)L.dtab;
   a.src = (const T *)L.src;
   a.dst = (T *)L.dst;
   a.rhs = (const T *)L.rhs;

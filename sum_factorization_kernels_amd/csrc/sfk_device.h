@@ -8,6 +8,8 @@
 
 #include <vector>
 
+#include "sfk_internal.h"
+
 namespace sfk {
 
 // Even-odd form of the contraction out[o] = sum_k K[k][o] in[k] of an N x N
@@ -52,8 +54,8 @@ inline Eo<N, T> make_eo(const std::vector<double> &Mat, bool forward)
   return e;
 }
 
-template <int N, int SIG, typename T>
-__device__ __forceinline__ void eo_apply(const Eo<N, T> &t, const T (&in)[N], T (&out)[N])
+template <int N, int SIG, typename E, typename T>
+__device__ __forceinline__ void eo_apply(const E &t, const T (&in)[N], T (&out)[N])
 {
   constexpr int mid = N / 2;
   T e[Eo<N, T>::M], o[Eo<N, T>::M];
@@ -97,6 +99,65 @@ struct Tab {
   T hd;             // phi_0'(0) of the Hermite-like basis
 };
 
+// The tables of an operator live in device memory (one copy per handle) and
+// the kernels read them through a constant-address-space pointer, so the
+// coefficients of each sweep are scalar loads right before the sweep.  (As
+// kernel arguments, or loaded once, they would all stay live in scalar
+// registers for the whole kernel and spill.)  opaque_ptr hides the pointer
+// value from the optimiser at every use site, which keeps it from hoisting
+// all table loads to the kernel entry.
+template <int N, typename T>
+struct DevTab {
+  Tab<N, T> t;
+  T Dd[N][N]; // dense collocation derivative D[m][q] (tangential derivatives)
+};
+
+#define SFK_CONST_AS __attribute__((address_space(4)))
+
+template <typename P>
+__device__ __forceinline__ P opaque_ptr(P p)
+{
+  __asm__ volatile("" : "+s"(p));
+  return p;
+}
+
+template <int N, typename T>
+__device__ __forceinline__ const SFK_CONST_AS Tab<N, T> &ctab(const DevTab<N, T> *p)
+{
+  return ((const SFK_CONST_AS DevTab<N, T> *)opaque_ptr(p))->t;
+}
+
+template <int N, typename T>
+__device__ __forceinline__ const SFK_CONST_AS DevTab<N, T> &cdevtab(const DevTab<N, T> *p)
+{
+  return *(const SFK_CONST_AS DevTab<N, T> *)opaque_ptr(p);
+}
+
+template <int N, typename T>
+inline DevTab<N, T> make_devtab(const Tables1D &t)
+{
+  DevTab<N, T> d{};
+  Tab<N, T> &tb = d.t;
+  tb.Sf = make_eo<N, T>(t.S, true);
+  tb.Sb = make_eo<N, T>(t.S, false);
+  tb.Df = make_eo<N, T>(t.D, true);
+  tb.Db = make_eo<N, T>(t.D, false);
+  if ((int)t.Sinv.size() == N * N) {
+    tb.If = make_eo<N, T>(t.Sinv, true);
+    tb.Ib = make_eo<N, T>(t.Sinv, false);
+  }
+  for (int i = 0; i < N; ++i) {
+    tb.fv0[i] = T(t.fv0[i]);
+    tb.fv1[i] = T(t.fv1[i]);
+    tb.fd0[i] = T(t.fd0[i]);
+    tb.fd1[i] = T(t.fd1[i]);
+    tb.w[i] = T(t.weights[i]);
+    for (int q = 0; q < N; ++q) d.Dd[i][q] = T(t.D[i * N + q]);
+  }
+  tb.hd = T(t.hermite_d);
+  return d;
+}
+
 // Line helpers inside one cell's LDS block (x fastest, row pitch NP).
 template <int N, int NP, int DIR>
 __device__ __forceinline__ int line_base(int l)
@@ -126,11 +187,12 @@ __device__ __forceinline__ void lds_write_line(T *cell, int l, const T (&v)[N])
   for (int i = 0; i < N; ++i) cell[b + i * line_stride<N, NP, DIR>()] = v[i];
 }
 
-// quadrature weight of point i on line l of direction DIR
-template <int N, int DIR, typename T>
-__device__ __forceinline__ T line_weight(const T *w, int l, int i)
+// quadrature weight of point i on line l of direction DIR (w: the device
+// table; the per-lane entries are vector loads)
+template <int N, int DIR, typename P>
+__device__ __forceinline__ auto line_weight(P w, int l, int i)
 {
-  return w[i] * w[l % N] * w[l / N];
+  return w[i] * (w[l % N] * w[l / N]);
 }
 
 // dof index (inside a cell) of point `layer` along DIR at face point fa

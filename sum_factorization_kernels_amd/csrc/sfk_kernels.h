@@ -40,6 +40,7 @@ struct CellLaunch {
   double load_value = 0.;
   int cont_scatter = -1;   // continuous: -1 automatic, 0 zero fill + atomics, 1 colored
   int cb = 0, order = 1;   // schedule
+  const void *dtab = nullptr; // DevTab<N,T> of the operator in device memory
 };
 
 hipError_t launch_cell(const CellLaunch &L, hipStream_t stream);
@@ -60,6 +61,7 @@ struct SipgLaunch {
   long long ncells = 0;
   int nc[3] = {1, 1, 1};
   int cb = 0, order = 1; // schedule
+  const void *dtab = nullptr;
 };
 hipError_t launch_sipg_general(const SipgLaunch &L, hipStream_t stream);
 
@@ -78,6 +80,7 @@ struct AdvectLaunch {
   int nc[3] = {1, 1, 1};
   double inv_jac[3] = {1., 1., 1.};
   int cb = 0, order = 1; // schedule
+  const void *dtab = nullptr;
 };
 hipError_t launch_advect(const AdvectLaunch &L, hipStream_t stream);
 
@@ -92,6 +95,7 @@ struct CoefGradLaunch {
   double inv_jac[3] = {1., 1., 1.};
   int nrhs = 1;
   long long vstride = 0;
+  const void *dtab = nullptr;
 };
 hipError_t launch_coef_gradient(const CoefGradLaunch &L, hipStream_t stream);
 
@@ -106,6 +110,10 @@ hipError_t vec_copy_convert(bool to_fp32, const void *x, void *y, long long n, h
 hipError_t vec_dot(bool fp32, const void *x, const void *y, long long n, double *result_dev,
                    double *partials, int n_partials, hipStream_t s);
 // y = a*x + b*y (FP64 scalars from device memory if ptrs given)
+// the operator's 1D tables (DevTab<n, T> of sfk_device.h) in a new device
+// allocation, T = float or double
+hipError_t upload_device_tables(const Tables1D &t, bool fp32, void **dev);
+
 // result_dev[0] = x.y, result_dev[1] = x.w (FP64 vectors; partials: 2*n_partials)
 hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, double *result_dev, double *partials,
                     int n_partials, hipStream_t s);

@@ -18,6 +18,7 @@ struct sfk_op {
   void *cell_vel = nullptr, *face_vel = nullptr;
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  void *dtab = nullptr; // 1D tables in device memory (uploaded on first launch)
   // workgroup schedule (sfk_schedule): cells per block (0 = default), block
   // order, continuous scatter (-1 automatic, 0 atomic, 1 colored)
   int sched_cb = 0, sched_order = 1, cont_scatter = -1;
@@ -33,6 +34,7 @@ inline void fill_sipg_launch(const sfk_op *op, SipgLaunch &L)
   for (int d = 0; d < 3; ++d) L.nc[d] = op->d.n_cells[d];
   L.cell_metric = op->cell_metric;
   L.face_metric = op->face_metric;
+  L.dtab = op->dtab;
   L.cb = op->sched_cb;
   L.order = op->sched_order;
 }
@@ -51,6 +53,7 @@ inline void fill_advect_launch(const sfk_op *op, AdvectLaunch &L)
     L.cell_vel = op->cell_vel;
     L.face_vel = op->face_vel;
   }
+  L.dtab = op->dtab;
   L.cb = op->sched_cb;
   L.order = op->sched_order;
 }

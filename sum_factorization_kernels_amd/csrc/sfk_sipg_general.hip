@@ -51,8 +51,7 @@ constexpr int gen_waves()
 
 template <int N, typename T>
 struct GenArgs {
-  Tab<N, T> tab;
-  T Dd[N][N]; // dense collocation derivative D[m][q] (tangential derivatives)
+  const DevTab<N, T> *dt; // tables in device memory (read through ctab/cdevtab)
   const T *src;
   T *dst;
   const T *rhs, *diag;
@@ -79,7 +78,8 @@ __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, c
                                            const T *F, int l, long long cell, T (&tv)[2], T (&td)[2][3])
 {
   constexpr int N2 = N * N;
-  const Tab<N, T> &tb = a.tab;
+  const auto &dtb = cdevtab(a.dt);
+  const auto &tb = dtb.t;
   T q[N], g[3][N];
   lds_read_line<N, N, DIR>(Q, l, q);
   lds_read_line<N, N, DIR>(Gx, l, g[0]);
@@ -89,8 +89,8 @@ __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, c
   const int llo = l % N, lhi = l / N;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    const T *fv = s ? tb.fv1 : tb.fv0;
-    const T *fd = s ? tb.fd1 : tb.fd0;
+    const auto *fv = s ? tb.fv1 : tb.fv0;
+    const auto *fd = s ? tb.fd1 : tb.fd0;
     T um = 0, gm[3] = {0, 0, 0};
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -106,8 +106,8 @@ __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, c
       gp[DIR] = F[(6 + 2 * DIR + s) * N2 + l];
 #pragma unroll
       for (int m = 0; m < N; ++m) {
-        gp[LO] += a.Dd[m][llo] * FV[lhi * N + m];
-        gp[HI] += a.Dd[m][lhi] * FV[m * N + llo];
+        gp[LO] += dtb.Dd[m][llo] * FV[lhi * N + m];
+        gp[HI] += dtb.Dd[m][lhi] * FV[m * N + llo];
       }
     }
     const long long owner = s ? cell : neighbor_cell(cell, a.nx, a.ny, a.nz, DIR, 0);
@@ -131,8 +131,8 @@ __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, c
 }
 
 // inject the gradient tests of direction DIR's faces into T (own d-line)
-template <int N, int DIR, typename T>
-__device__ __forceinline__ void inject_grad(const Tab<N, T> &tb, T *Tx, T *Ty, T *Tz, int l, const T (&td)[2][3])
+template <int N, int DIR, typename T, typename E>
+__device__ __forceinline__ void inject_grad(const E &tb, T *Tx, T *Ty, T *Tz, int l, const T (&td)[2][3])
 {
   T *Ts[3] = {Tx, Ty, Tz};
 #pragma unroll
@@ -155,7 +155,6 @@ sipg_general_kernel(const GenArgs<N, T> a)
   T *sQ = dyn_lds<T>();
   T *sG[3] = {sQ + CB * N3, sQ + 2 * CB * N3, sQ + 3 * CB * N3};
   T *sF = sQ + 4 * CB * N3;
-  const Tab<N, T> &tb = a.tab;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
   const long long item0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB, item = item0 + c;
   const bool act = c < CB && item < a.nitems;
@@ -175,7 +174,7 @@ sipg_general_kernel(const GenArgs<N, T> a)
       if constexpr (DIAG) u[k] = (k * N2 + l == unit) ? T(1) : T(0);
       else u[k] = a.src[cell * N3 + k * N2 + l];
     }
-    eo_apply<N, 1>(tb.Sf, u, v);
+    eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
     lds_write_line<N, N, 2>(Q, l, v);
   }
   if constexpr (!DIAG) {
@@ -199,7 +198,7 @@ sipg_general_kernel(const GenArgs<N, T> a)
       }
       const T uf = a.src[nb + of], ui = a.src[nb + oi];
       sF[(cc * 12 + f) * N2 + fa] = uf;
-      sF[(cc * 12 + 6 + f) * N2 + fa] = (s ? -tb.hd : tb.hd) * (ui - uf);
+      sF[(cc * 12 + 6 + f) * N2 + fa] = (s ? -ctab(a.dt).hd : ctab(a.dt).hd) * (ui - uf);
     }
   }
   __syncthreads();
@@ -207,14 +206,14 @@ sipg_general_kernel(const GenArgs<N, T> a)
   if (act) {
     T u[N], v[N];
     lds_read_line<N, N, 0>(Q, l, u);
-    eo_apply<N, 1>(tb.Sf, u, v);
+    eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
     lds_write_line<N, N, 0>(Q, l, v);
     if constexpr (!DIAG) {
       for (int it = l; it < 12 * N; it += N2) {
         T *fl = F + (it / N) * N2 + (it % N) * N;
 #pragma unroll
         for (int i = 0; i < N; ++i) u[i] = fl[i];
-        eo_apply<N, 1>(tb.Sf, u, v);
+        eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
 #pragma unroll
         for (int i = 0; i < N; ++i) fl[i] = v[i];
       }
@@ -225,14 +224,14 @@ sipg_general_kernel(const GenArgs<N, T> a)
   if (act) {
     T u[N], v[N];
     lds_read_line<N, N, 1>(Q, l, u);
-    eo_apply<N, 1>(tb.Sf, u, v);
+    eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
     lds_write_line<N, N, 1>(Q, l, v);
     if constexpr (!DIAG) {
       for (int it = l; it < 12 * N; it += N2) {
         T *fl = F + (it / N) * N2 + (it % N);
 #pragma unroll
         for (int i = 0; i < N; ++i) u[i] = fl[i * N];
-        eo_apply<N, 1>(tb.Sf, u, v);
+        eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
 #pragma unroll
         for (int i = 0; i < N; ++i) fl[i * N] = v[i];
       }
@@ -243,13 +242,13 @@ sipg_general_kernel(const GenArgs<N, T> a)
   if (act) {
     T u[N], v[N];
     lds_read_line<N, N, 0>(Q, l, u);
-    eo_apply<N, -1>(tb.Df, u, v);
+    eo_apply<N, -1>(ctab(a.dt).Df, u, v);
     lds_write_line<N, N, 0>(Gx, l, v);
     lds_read_line<N, N, 1>(Q, l, u);
-    eo_apply<N, -1>(tb.Df, u, v);
+    eo_apply<N, -1>(ctab(a.dt).Df, u, v);
     lds_write_line<N, N, 1>(Gy, l, v);
     lds_read_line<N, N, 2>(Q, l, u);
-    eo_apply<N, -1>(tb.Df, u, v);
+    eo_apply<N, -1>(ctab(a.dt).Df, u, v);
     lds_write_line<N, N, 2>(Gz, l, v);
   }
   __syncthreads();
@@ -274,9 +273,9 @@ sipg_general_kernel(const GenArgs<N, T> a)
       const T g00 = load_stream(m + q), g11 = load_stream(m + N3 + q), g22 = load_stream(m + 2 * N3 + q),
               g01 = load_stream(m + 3 * N3 + q), g02 = load_stream(m + 4 * N3 + q), g12 = load_stream(m + 5 * N3 + q);
       const T x = gx[k], y = gy[k], z = gz[k];
-      gx[k] = g00 * x + g01 * y + g02 * z + tb.fv0[k] * tdz[0][0] + tb.fv1[k] * tdz[1][0];
-      gy[k] = g01 * x + g11 * y + g12 * z + tb.fv0[k] * tdz[0][1] + tb.fv1[k] * tdz[1][1];
-      gz[k] = g02 * x + g12 * y + g22 * z + tb.fv0[k] * tdz[0][2] + tb.fv1[k] * tdz[1][2];
+      gx[k] = g00 * x + g01 * y + g02 * z + ctab(a.dt).fv0[k] * tdz[0][0] + ctab(a.dt).fv1[k] * tdz[1][0];
+      gy[k] = g01 * x + g11 * y + g12 * z + ctab(a.dt).fv0[k] * tdz[0][1] + ctab(a.dt).fv1[k] * tdz[1][1];
+      gz[k] = g02 * x + g12 * y + g22 * z + ctab(a.dt).fv0[k] * tdz[0][2] + ctab(a.dt).fv1[k] * tdz[1][2];
     }
     lds_write_line<N, N, 2>(Gx, l, gx);
     lds_write_line<N, N, 2>(Gy, l, gy);
@@ -284,17 +283,17 @@ sipg_general_kernel(const GenArgs<N, T> a)
   }
   __syncthreads();
   // ----------------------------------------------------------------- Iy
-  if (act) inject_grad<N, 1, T>(tb, Gx, Gy, Gz, l, tdy);
+  if (act) inject_grad<N, 1, T>(ctab(a.dt), Gx, Gy, Gz, l, tdy);
   __syncthreads();
   // ----------------------------------------------------------------- Ix
   if (act) {
     T v[N], r[N];
     lds_read_line<N, N, 0>(Gx, l, v);
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += tb.fv0[i] * tdx[0][0] + tb.fv1[i] * tdx[1][0];
-    eo_apply<N, -1>(tb.Db, v, r);
+    for (int i = 0; i < N; ++i) v[i] += ctab(a.dt).fv0[i] * tdx[0][0] + ctab(a.dt).fv1[i] * tdx[1][0];
+    eo_apply<N, -1>(ctab(a.dt).Db, v, r);
 #pragma unroll
-    for (int i = 0; i < N; ++i) r[i] += tb.fv0[i] * tvx[0] + tb.fv1[i] * tvx[1];
+    for (int i = 0; i < N; ++i) r[i] += ctab(a.dt).fv0[i] * tvx[0] + ctab(a.dt).fv1[i] * tvx[1];
     lds_write_line<N, N, 0>(Q, l, r);
     // tangential x-face tests into T_y, T_z
 #pragma unroll
@@ -302,7 +301,7 @@ sipg_general_kernel(const GenArgs<N, T> a)
       T *Te = e == 1 ? Gy : Gz;
       lds_read_line<N, N, 0>(Te, l, v);
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] += tb.fv0[i] * tdx[0][e] + tb.fv1[i] * tdx[1][e];
+      for (int i = 0; i < N; ++i) v[i] += ctab(a.dt).fv0[i] * tdx[0][e] + ctab(a.dt).fv1[i] * tdx[1][e];
       lds_write_line<N, N, 0>(Te, l, v);
     }
   }
@@ -311,10 +310,10 @@ sipg_general_kernel(const GenArgs<N, T> a)
   if (act) {
     T v[N], r[N], b[N];
     lds_read_line<N, N, 1>(Gy, l, v);
-    eo_apply<N, -1>(tb.Db, v, r);
+    eo_apply<N, -1>(ctab(a.dt).Db, v, r);
     lds_read_line<N, N, 1>(Q, l, b);
 #pragma unroll
-    for (int i = 0; i < N; ++i) r[i] += b[i] + tb.fv0[i] * tvy[0] + tb.fv1[i] * tvy[1];
+    for (int i = 0; i < N; ++i) r[i] += b[i] + ctab(a.dt).fv0[i] * tvy[0] + ctab(a.dt).fv1[i] * tvy[1];
     lds_write_line<N, N, 1>(Q, l, r);
   }
   __syncthreads();
@@ -322,11 +321,11 @@ sipg_general_kernel(const GenArgs<N, T> a)
   if (act) {
     T v[N], r[N], b[N];
     lds_read_line<N, N, 2>(Gz, l, v);
-    eo_apply<N, -1>(tb.Db, v, r);
+    eo_apply<N, -1>(ctab(a.dt).Db, v, r);
     lds_read_line<N, N, 2>(Q, l, b);
 #pragma unroll
-    for (int i = 0; i < N; ++i) r[i] += b[i] + tb.fv0[i] * tvz[0] + tb.fv1[i] * tvz[1];
-    eo_apply<N, 1>(tb.Sb, r, v);
+    for (int i = 0; i < N; ++i) r[i] += b[i] + ctab(a.dt).fv0[i] * tvz[0] + ctab(a.dt).fv1[i] * tvz[1];
+    eo_apply<N, 1>(ctab(a.dt).Sb, r, v);
     lds_write_line<N, N, 2>(Q, l, v);
   }
   __syncthreads();
@@ -334,7 +333,7 @@ sipg_general_kernel(const GenArgs<N, T> a)
   if (act) {
     T u[N], v[N];
     lds_read_line<N, N, 1>(Q, l, u);
-    eo_apply<N, 1>(tb.Sb, u, v);
+    eo_apply<N, 1>(ctab(a.dt).Sb, u, v);
     lds_write_line<N, N, 1>(Q, l, v);
   }
   __syncthreads();
@@ -342,7 +341,7 @@ sipg_general_kernel(const GenArgs<N, T> a)
   if (act) {
     T u[N], o[N];
     lds_read_line<N, N, 0>(Q, l, u);
-    eo_apply<N, 1>(tb.Sb, u, o);
+    eo_apply<N, 1>(ctab(a.dt).Sb, u, o);
     if constexpr (DIAG) {
       if (unit / N == l) a.dst[cell * N3 + unit] = o[unit % N];
     } else {
@@ -367,21 +366,8 @@ sipg_general_kernel(const GenArgs<N, T> a)
 template <int N, typename T>
 static hipError_t launch_general_n(const SipgLaunch &L, hipStream_t s)
 {
-  const Tables1D &t = *L.tables;
   GenArgs<N, T> a{};
-  a.tab.Sf = make_eo<N, T>(t.S, true);
-  a.tab.Sb = make_eo<N, T>(t.S, false);
-  a.tab.Df = make_eo<N, T>(t.D, true);
-  a.tab.Db = make_eo<N, T>(t.D, false);
-  for (int i = 0; i < N; ++i) {
-    a.tab.fv0[i] = T(t.fv0[i]);
-    a.tab.fv1[i] = T(t.fv1[i]);
-    a.tab.fd0[i] = T(t.fd0[i]);
-    a.tab.fd1[i] = T(t.fd1[i]);
-    a.tab.w[i] = T(t.weights[i]);
-    for (int q = 0; q < N; ++q) a.Dd[i][q] = T(t.D[i * N + q]);
-  }
-  a.tab.hd = T(t.hermite_d);
+  a.dt = (const DevTab<N, T> *)L.dtab;
   a.src = (const T *)L.src;
   a.dst = (T *)L.dst;
   a.rhs = (const T *)L.rhs;

@@ -239,6 +239,29 @@ hipError_t layout_convert(int degree, const int nc[3], int lanes, const double *
   return hipGetLastError();
 }
 
+template <int N, typename T>
+static hipError_t upload_devtab(const Tables1D &t, void **dev)
+{
+  const DevTab<N, T> h = make_devtab<N, T>(t);
+  hipError_t e = hipMalloc(dev, sizeof(h));
+  if (e != hipSuccess) {
+    *dev = nullptr;
+    return e;
+  }
+  return hipMemcpy(*dev, &h, sizeof(h), hipMemcpyHostToDevice);
+}
+
+hipError_t upload_device_tables(const Tables1D &t, bool fp32, void **dev)
+{
+  switch (t.n) {
+#define SFK_CASE(NN) \
+  case NN: return fp32 ? upload_devtab<NN, float>(t, dev) : upload_devtab<NN, double>(t, dev);
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t apply_1d(int nn, const std::vector<double> &M, int sym, bool forward, long long n_lines,
                     long long line_stride, int stride, const double *in, double *out, hipStream_t s)
 {
