@@ -179,41 +179,46 @@ advect_kernel(const AdvArgs<N, T> a)
   if constexpr (KIND == 2) {
     // M^{-1} x = Sinv^T W^{-1} Sinv x
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(ctab(a.dt).Ib, u, v);
+      eo_apply<N, 1>(tb.Ib, u, v);
       lds_write_line<N, N, 2>(A, l, v);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Ib, u, v);
+      eo_apply<N, 1>(tb.Ib, u, v);
       lds_write_line<N, N, 0>(A, l, v);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Ib, u, v);
+      eo_apply<N, 1>(tb.Ib, u, v);
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] /= a.jxw * line_weight<N, 1>(ctab(a.dt).w, l, i);
-      eo_apply<N, 1>(ctab(a.dt).If, v, u);
+      for (int i = 0; i < N; ++i) v[i] /= a.jxw * line_weight<N, 1>(tb.w, l, i);
+      eo_apply<N, 1>(tb.If, v, u);
       lds_write_line<N, N, 1>(A, l, u);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).If, u, v);
+      eo_apply<N, 1>(tb.If, u, v);
       lds_write_line<N, N, 0>(A, l, v);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 2>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).If, u, v);
+      eo_apply<N, 1>(tb.If, u, v);
 #pragma unroll
       for (int k = 0; k < N; ++k) a.dst[cell * N3 + k * N2 + l] = v[k];
     }
@@ -222,14 +227,15 @@ advect_kernel(const AdvArgs<N, T> a)
     // velocity gradient: A <- u at q-points, B <- mu at q-points,
     // F[0..2] <- u^+ on the upper faces, F[3..5] <- mu^+ on the upper faces
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 2>(A, l, v);
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.aux[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 2>(B, l, v);
     }
     for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
@@ -247,52 +253,55 @@ advect_kernel(const AdvArgs<N, T> a)
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 0>(A, l, v);
       lds_read_line<N, N, 0>(B, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 0>(B, l, v);
-      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, true);
+      face_sweep_t<N, T>(tb.Sf, F, 6, l, true);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 1>(A, l, v);
       lds_read_line<N, N, 1>(B, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 1>(B, l, v);
-      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, false);
+      face_sweep_t<N, T>(tb.Sf, F, 6, l, false);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       auto do_dir = [&](auto dirc) {
         constexpr int DIR = decltype(dirc)::value;
         T uq[N], mq[N], g[N];
         lds_read_line<N, N, DIR>(A, l, uq);
         lds_read_line<N, N, DIR>(B, l, mq);
-        eo_apply<N, -1>(ctab(a.dt).Df, mq, g);
+        eo_apply<N, -1>(tb.Df, mq, g);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           const int dof = line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
           a.dst[(cell * 3 + DIR) * N3 + dof] =
-              g[i] * a.ij[DIR] * uq[i] * (a.jxw * line_weight<N, DIR>(ctab(a.dt).w, l, i));
+              g[i] * a.ij[DIR] * uq[i] * (a.jxw * line_weight<N, DIR>(tb.w, l, i));
         }
         T um = 0, mm = 0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          um += ctab(a.dt).fv1[i] * uq[i];
-          mm += ctab(a.dt).fv1[i] * mq[i];
+          um += tb.fv1[i] * uq[i];
+          mm += tb.fv1[i] * mq[i];
         }
         const T up = F[DIR * N2 + l], mp = F[(3 + DIR) * N2 + l];
         const T b = a.face_vel[((long long)DIR * a.ncells + cell) * N2 + l];
         const T sg = b > T(0) ? T(1) : (b < T(0) ? T(-1) : T(0));
         const T uh = T(0.5) * (um + up) + T(0.5) * sg * (um - up);
         a.dst2[((long long)DIR * a.ncells + cell) * N2 + l] =
-            -(mm - mp) * uh * (ctab(a.dt).w[l % N] * ctab(a.dt).w[l / N] * a.jxwf[DIR]);
+            -(mm - mp) * uh * (tb.w[l % N] * tb.w[l / N] * a.jxwf[DIR]);
       };
       do_dir(std::integral_constant<int, 0>());
       do_dir(std::integral_constant<int, 1>());
@@ -302,37 +311,42 @@ advect_kernel(const AdvArgs<N, T> a)
   } else {
     // ------------------------------------------------ L: z values + faces
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 2>(A, l, v);
     }
     load_face_layers<N, T>(a, a.src, sF, cell0, CB, 0, 6);
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 0>(A, l, v);
-      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, true);
+      face_sweep_t<N, T>(tb.Sf, F, 6, l, true);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 1>(A, l, v);
-      face_sweep_t<N, T>(ctab(a.dt).Sf, F, 6, l, false);
+      face_sweep_t<N, T>(tb.Sf, F, 6, l, false);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T r[N];
       adv_dir<N, 0, T, VAR, ADJ>(a, A, F, l, cell, r);
       lds_write_line<N, N, 0>(B, l, r);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T r[N], b[N];
       adv_dir<N, 1, T, VAR, ADJ>(a, A, F, l, cell, r);
       lds_read_line<N, N, 1>(B, l, b);
@@ -342,6 +356,7 @@ advect_kernel(const AdvArgs<N, T> a)
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T r[N], b[N], o[N];
       adv_dir<N, 2, T, VAR, ADJ>(a, A, F, l, cell, r);
       lds_read_line<N, N, 2>(B, l, b);
@@ -350,27 +365,29 @@ advect_kernel(const AdvArgs<N, T> a)
       if constexpr (!ADJ) {
         // inverse mass matrix by collocation, evaluation_dg_advect.h:619-624
 #pragma unroll
-        for (int i = 0; i < N; ++i) r[i] /= a.jxw * line_weight<N, 2>(ctab(a.dt).w, l, i);
-        eo_apply<N, 1>(ctab(a.dt).If, r, o);
+        for (int i = 0; i < N; ++i) r[i] /= a.jxw * line_weight<N, 2>(tb.w, l, i);
+        eo_apply<N, 1>(tb.If, r, o);
       } else {
-        eo_apply<N, 1>(ctab(a.dt).Sb, r, o);
+        eo_apply<N, 1>(tb.Sb, r, o);
       }
       lds_write_line<N, N, 2>(A, l, o);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      if constexpr (!ADJ) eo_apply<N, 1>(ctab(a.dt).If, u, v);
-      else eo_apply<N, 1>(ctab(a.dt).Sb, u, v);
+      if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, v);
+      else eo_apply<N, 1>(tb.Sb, u, v);
       lds_write_line<N, N, 1>(A, l, v);
     }
     __syncthreads();
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], o[N];
       lds_read_line<N, N, 0>(A, l, u);
-      if constexpr (!ADJ) eo_apply<N, 1>(ctab(a.dt).If, u, o);
-      else eo_apply<N, 1>(ctab(a.dt).Sb, u, o);
+      if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, o);
+      else eo_apply<N, 1>(tb.Sb, u, o);
       const long long g0 = cell * N3 + l * N;
       if constexpr (STAGE) {
         // evaluation_dg_advect.h:651-667

@@ -183,6 +183,12 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
   }
   op->ncq = op->ncells * n * n * n;
   op->nfq = op->ncells * n * n;
+  // tables to the device now when a device is present (a first launch
+  // inside a graph capture must not allocate); without one the upload
+  // waits for the first launch
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) (void)ensure_device_tables(op);
+  (void)hipGetLastError();
   *out = op;
   return SFK_OK;
 }

@@ -222,6 +222,7 @@ cell_kernel(const CellArgs<N, T> a)
     T *dst = a.dst + rr * a.vstride;
     // ---------------------------------------------------------------- L
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       if constexpr (MODE == Mode::Cont) {
         const int X = ccx * p + l % N, Y = ccy * p + l / N;
@@ -237,7 +238,7 @@ cell_kernel(const CellArgs<N, T> a)
 #pragma unroll
         for (int k = 0; k < N; ++k) u[k] = src[cell * N3 + k * N2 + l];
       }
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 2>(A, l, v);
     }
     if constexpr (FACES) {
@@ -262,7 +263,7 @@ cell_kernel(const CellArgs<N, T> a)
           oi = face_dof<N, 2>(li, fa);
         }
         const T uf = src[nb + of], ui = src[nb + oi];
-        const T w0 = s ? -ctab(a.dt).hd : ctab(a.dt).hd;
+        const T hd = ctab(a.dt).hd, w0 = s ? -hd : hd;
         sF[(cc * 12 + f) * N2 + fa] = uf;
         sF[(cc * 12 + 6 + f) * N2 + fa] = w0 * (ui - uf);
       }
@@ -270,16 +271,17 @@ cell_kernel(const CellArgs<N, T> a)
     __syncthreads();
     // --------------------------------------------------------------- V1
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 0>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 0>(A, l, v);
       if constexpr (FACES) {
         for (int it = l; it < 12 * N; it += N2) {
           T *fl = F + (it / N) * N2 + (it % N) * N; // lo-direction line
 #pragma unroll
           for (int i = 0; i < N; ++i) u[i] = fl[i];
-          eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+          eo_apply<N, 1>(tb.Sf, u, v);
 #pragma unroll
           for (int i = 0; i < N; ++i) fl[i] = v[i];
         }
@@ -288,16 +290,17 @@ cell_kernel(const CellArgs<N, T> a)
     __syncthreads();
     // --------------------------------------------------------------- V2
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+      eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 1>(A, l, v);
       if constexpr (FACES) {
         for (int it = l; it < 12 * N; it += N2) {
           T *fl = F + (it / N) * N2 + (it % N); // hi-direction line
 #pragma unroll
           for (int i = 0; i < N; ++i) u[i] = fl[i * N];
-          eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+          eo_apply<N, 1>(tb.Sf, u, v);
 #pragma unroll
           for (int i = 0; i < N; ++i) fl[i * N] = v[i];
         }
@@ -306,6 +309,7 @@ cell_kernel(const CellArgs<N, T> a)
     __syncthreads();
     // --------------------------------------------------------------- Gx
     if (act) {
+      const auto &tb = ctab(a.dt);
       T r[N];
       grad_phase<N, 0, T, MODE, COEF>(a, A, F, l, cell, r);
       lds_write_line<N, N, 0>(B, l, r);
@@ -313,6 +317,7 @@ cell_kernel(const CellArgs<N, T> a)
     __syncthreads();
     // --------------------------------------------------------------- Gy
     if (act) {
+      const auto &tb = ctab(a.dt);
       T r[N], b[N];
       grad_phase<N, 1, T, MODE, COEF>(a, A, F, l, cell, r);
       lds_read_line<N, N, 1>(B, l, b);
@@ -323,32 +328,35 @@ cell_kernel(const CellArgs<N, T> a)
     __syncthreads();
     // --------------------------------------------------------------- Gz
     if (act) {
+      const auto &tb = ctab(a.dt);
       T r[N], b[N], o[N];
       grad_phase<N, 2, T, MODE, COEF>(a, A, F, l, cell, r);
       lds_read_line<N, N, 2>(B, l, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) r[i] += b[i];
-      eo_apply<N, 1>(ctab(a.dt).Sb, r, o);
+      eo_apply<N, 1>(tb.Sb, r, o);
       lds_write_line<N, N, 2>(A, l, o);
     }
     __syncthreads();
     // --------------------------------------------------------------- By
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
-      eo_apply<N, 1>(ctab(a.dt).Sb, u, v);
+      eo_apply<N, 1>(tb.Sb, u, v);
       lds_write_line<N, N, 1>(A, l, v);
     }
     __syncthreads();
     // --------------------------------------------------------------- Bx
     T o[N];
     if (act) {
+      const auto &tb = ctab(a.dt);
       T u[N];
       lds_read_line<N, N, 0>(A, l, u);
       if constexpr (REFX)
-        eo_apply<N, 1>(ctab(a.dt).Sf, u, o); // evaluation_dg_laplacian.h:481-483
+        eo_apply<N, 1>(tb.Sf, u, o); // evaluation_dg_laplacian.h:481-483
       else
-        eo_apply<N, 1>(ctab(a.dt).Sb, u, o);
+        eo_apply<N, 1>(tb.Sb, u, o);
       if constexpr (MODE == Mode::Cont) {
         // x-merge: the first x-plane of cell c+1 is the last one of cell c
         if (a.color >= 0) B[l] = o[0];
@@ -370,6 +378,7 @@ cell_kernel(const CellArgs<N, T> a)
       }
     }
     if (act) {
+      const auto &tb = ctab(a.dt);
       if constexpr (MODE == Mode::Cont) {
       } else if constexpr (CHEBY) {
         // r = A u - rhs; tm = c_tm*tm + c_np*diag*r; dst = u - tm

@@ -75,11 +75,11 @@ __device__ __forceinline__ int pt(int l, int i)
 // traces, neighbour data and face metric of one direction -> test coefficients
 template <int N, int DIR, typename T, bool DIAG>
 __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, const T *Gx, const T *Gy, const T *Gz,
-                                           const T *F, int l, long long cell, T (&tv)[2], T (&td)[2][3])
+                                           const T *F, int l, long long cell, const T (&dlo)[N], const T (&dhi)[N],
+                                           T (&tv)[2], T (&td)[2][3])
 {
   constexpr int N2 = N * N;
-  const auto &dtb = cdevtab(a.dt);
-  const auto &tb = dtb.t;
+  const auto &tb = ctab(a.dt);
   T q[N], g[3][N];
   lds_read_line<N, N, DIR>(Q, l, q);
   lds_read_line<N, N, DIR>(Gx, l, g[0]);
@@ -106,8 +106,8 @@ __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, c
       gp[DIR] = F[(6 + 2 * DIR + s) * N2 + l];
 #pragma unroll
       for (int m = 0; m < N; ++m) {
-        gp[LO] += dtb.Dd[m][llo] * FV[lhi * N + m];
-        gp[HI] += dtb.Dd[m][lhi] * FV[m * N + llo];
+        gp[LO] += dlo[m] * FV[lhi * N + m];
+        gp[HI] += dhi[m] * FV[m * N + llo];
       }
     }
     const long long owner = s ? cell : neighbor_cell(cell, a.nx, a.ny, a.nz, DIR, 0);
@@ -168,13 +168,14 @@ sipg_general_kernel(const GenArgs<N, T> a)
 
   // ------------------------------------------------------------------ L
   if (act) {
+    const auto &tb = ctab(a.dt);
     T u[N], v[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       if constexpr (DIAG) u[k] = (k * N2 + l == unit) ? T(1) : T(0);
       else u[k] = a.src[cell * N3 + k * N2 + l];
     }
-    eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+    eo_apply<N, 1>(tb.Sf, u, v);
     lds_write_line<N, N, 2>(Q, l, v);
   }
   if constexpr (!DIAG) {
@@ -198,22 +199,24 @@ sipg_general_kernel(const GenArgs<N, T> a)
       }
       const T uf = a.src[nb + of], ui = a.src[nb + oi];
       sF[(cc * 12 + f) * N2 + fa] = uf;
-      sF[(cc * 12 + 6 + f) * N2 + fa] = (s ? -ctab(a.dt).hd : ctab(a.dt).hd) * (ui - uf);
+      const T hd = ctab(a.dt).hd;
+      sF[(cc * 12 + 6 + f) * N2 + fa] = (s ? -hd : hd) * (ui - uf);
     }
   }
   __syncthreads();
   // ----------------------------------------------------------------- V1
   if (act) {
+    const auto &tb = ctab(a.dt);
     T u[N], v[N];
     lds_read_line<N, N, 0>(Q, l, u);
-    eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+    eo_apply<N, 1>(tb.Sf, u, v);
     lds_write_line<N, N, 0>(Q, l, v);
     if constexpr (!DIAG) {
       for (int it = l; it < 12 * N; it += N2) {
         T *fl = F + (it / N) * N2 + (it % N) * N;
 #pragma unroll
         for (int i = 0; i < N; ++i) u[i] = fl[i];
-        eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+        eo_apply<N, 1>(tb.Sf, u, v);
 #pragma unroll
         for (int i = 0; i < N; ++i) fl[i] = v[i];
       }
@@ -222,16 +225,17 @@ sipg_general_kernel(const GenArgs<N, T> a)
   __syncthreads();
   // ----------------------------------------------------------------- V2
   if (act) {
+    const auto &tb = ctab(a.dt);
     T u[N], v[N];
     lds_read_line<N, N, 1>(Q, l, u);
-    eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+    eo_apply<N, 1>(tb.Sf, u, v);
     lds_write_line<N, N, 1>(Q, l, v);
     if constexpr (!DIAG) {
       for (int it = l; it < 12 * N; it += N2) {
         T *fl = F + (it / N) * N2 + (it % N);
 #pragma unroll
         for (int i = 0; i < N; ++i) u[i] = fl[i * N];
-        eo_apply<N, 1>(ctab(a.dt).Sf, u, v);
+        eo_apply<N, 1>(tb.Sf, u, v);
 #pragma unroll
         for (int i = 0; i < N; ++i) fl[i * N] = v[i];
       }
@@ -240,28 +244,43 @@ sipg_general_kernel(const GenArgs<N, T> a)
   __syncthreads();
   // ------------------------------------------------------------------ G
   if (act) {
+    const auto &tb = ctab(a.dt);
     T u[N], v[N];
     lds_read_line<N, N, 0>(Q, l, u);
-    eo_apply<N, -1>(ctab(a.dt).Df, u, v);
+    eo_apply<N, -1>(tb.Df, u, v);
     lds_write_line<N, N, 0>(Gx, l, v);
     lds_read_line<N, N, 1>(Q, l, u);
-    eo_apply<N, -1>(ctab(a.dt).Df, u, v);
+    eo_apply<N, -1>(tb.Df, u, v);
     lds_write_line<N, N, 1>(Gy, l, v);
     lds_read_line<N, N, 2>(Q, l, u);
-    eo_apply<N, -1>(ctab(a.dt).Df, u, v);
+    eo_apply<N, -1>(tb.Df, u, v);
     lds_write_line<N, N, 2>(Gz, l, v);
   }
   __syncthreads();
   // ------------------------------------------------------------------ E
   T tvx[2], tdx[2][3], tvy[2], tdy[2][3], tvz[2], tdz[2][3];
   if (act) {
-    face_terms<N, 0, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, tvx, tdx);
-    face_terms<N, 1, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, tvy, tdy);
-    face_terms<N, 2, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, tvz, tdz);
+    const auto &tb = ctab(a.dt);
+    // collocation-derivative entries D[m][l % N], D[m][l / N] of this
+    // thread's in-face indices (tangential derivatives of the neighbour's
+    // face data), the same for all three face directions
+    T dlo[N], dhi[N];
+    if constexpr (!DIAG) {
+      const auto &dtb = cdevtab(a.dt);
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        dlo[m] = dtb.Dd[m][l % N];
+        dhi[m] = dtb.Dd[m][l / N];
+      }
+    }
+    face_terms<N, 0, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, dlo, dhi, tvx, tdx);
+    face_terms<N, 1, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, dlo, dhi, tvy, tdy);
+    face_terms<N, 2, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, dlo, dhi, tvz, tdz);
   }
   __syncthreads();
   // ------------------------------------------------------------------ M
   if (act) {
+    const auto &tb = ctab(a.dt);
     T gx[N], gy[N], gz[N];
     lds_read_line<N, N, 2>(Gx, l, gx);
     lds_read_line<N, N, 2>(Gy, l, gy);
@@ -273,9 +292,9 @@ sipg_general_kernel(const GenArgs<N, T> a)
       const T g00 = load_stream(m + q), g11 = load_stream(m + N3 + q), g22 = load_stream(m + 2 * N3 + q),
               g01 = load_stream(m + 3 * N3 + q), g02 = load_stream(m + 4 * N3 + q), g12 = load_stream(m + 5 * N3 + q);
       const T x = gx[k], y = gy[k], z = gz[k];
-      gx[k] = g00 * x + g01 * y + g02 * z + ctab(a.dt).fv0[k] * tdz[0][0] + ctab(a.dt).fv1[k] * tdz[1][0];
-      gy[k] = g01 * x + g11 * y + g12 * z + ctab(a.dt).fv0[k] * tdz[0][1] + ctab(a.dt).fv1[k] * tdz[1][1];
-      gz[k] = g02 * x + g12 * y + g22 * z + ctab(a.dt).fv0[k] * tdz[0][2] + ctab(a.dt).fv1[k] * tdz[1][2];
+      gx[k] = g00 * x + g01 * y + g02 * z + tb.fv0[k] * tdz[0][0] + tb.fv1[k] * tdz[1][0];
+      gy[k] = g01 * x + g11 * y + g12 * z + tb.fv0[k] * tdz[0][1] + tb.fv1[k] * tdz[1][1];
+      gz[k] = g02 * x + g12 * y + g22 * z + tb.fv0[k] * tdz[0][2] + tb.fv1[k] * tdz[1][2];
     }
     lds_write_line<N, N, 2>(Gx, l, gx);
     lds_write_line<N, N, 2>(Gy, l, gy);
@@ -287,13 +306,14 @@ sipg_general_kernel(const GenArgs<N, T> a)
   __syncthreads();
   // ----------------------------------------------------------------- Ix
   if (act) {
+    const auto &tb = ctab(a.dt);
     T v[N], r[N];
     lds_read_line<N, N, 0>(Gx, l, v);
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += ctab(a.dt).fv0[i] * tdx[0][0] + ctab(a.dt).fv1[i] * tdx[1][0];
-    eo_apply<N, -1>(ctab(a.dt).Db, v, r);
+    for (int i = 0; i < N; ++i) v[i] += tb.fv0[i] * tdx[0][0] + tb.fv1[i] * tdx[1][0];
+    eo_apply<N, -1>(tb.Db, v, r);
 #pragma unroll
-    for (int i = 0; i < N; ++i) r[i] += ctab(a.dt).fv0[i] * tvx[0] + ctab(a.dt).fv1[i] * tvx[1];
+    for (int i = 0; i < N; ++i) r[i] += tb.fv0[i] * tvx[0] + tb.fv1[i] * tvx[1];
     lds_write_line<N, N, 0>(Q, l, r);
     // tangential x-face tests into T_y, T_z
 #pragma unroll
@@ -301,47 +321,51 @@ sipg_general_kernel(const GenArgs<N, T> a)
       T *Te = e == 1 ? Gy : Gz;
       lds_read_line<N, N, 0>(Te, l, v);
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] += ctab(a.dt).fv0[i] * tdx[0][e] + ctab(a.dt).fv1[i] * tdx[1][e];
+      for (int i = 0; i < N; ++i) v[i] += tb.fv0[i] * tdx[0][e] + tb.fv1[i] * tdx[1][e];
       lds_write_line<N, N, 0>(Te, l, v);
     }
   }
   __syncthreads();
   // ----------------------------------------------------------------- By
   if (act) {
+    const auto &tb = ctab(a.dt);
     T v[N], r[N], b[N];
     lds_read_line<N, N, 1>(Gy, l, v);
-    eo_apply<N, -1>(ctab(a.dt).Db, v, r);
+    eo_apply<N, -1>(tb.Db, v, r);
     lds_read_line<N, N, 1>(Q, l, b);
 #pragma unroll
-    for (int i = 0; i < N; ++i) r[i] += b[i] + ctab(a.dt).fv0[i] * tvy[0] + ctab(a.dt).fv1[i] * tvy[1];
+    for (int i = 0; i < N; ++i) r[i] += b[i] + tb.fv0[i] * tvy[0] + tb.fv1[i] * tvy[1];
     lds_write_line<N, N, 1>(Q, l, r);
   }
   __syncthreads();
   // ----------------------------------------------------------------- Bz
   if (act) {
+    const auto &tb = ctab(a.dt);
     T v[N], r[N], b[N];
     lds_read_line<N, N, 2>(Gz, l, v);
-    eo_apply<N, -1>(ctab(a.dt).Db, v, r);
+    eo_apply<N, -1>(tb.Db, v, r);
     lds_read_line<N, N, 2>(Q, l, b);
 #pragma unroll
-    for (int i = 0; i < N; ++i) r[i] += b[i] + ctab(a.dt).fv0[i] * tvz[0] + ctab(a.dt).fv1[i] * tvz[1];
-    eo_apply<N, 1>(ctab(a.dt).Sb, r, v);
+    for (int i = 0; i < N; ++i) r[i] += b[i] + tb.fv0[i] * tvz[0] + tb.fv1[i] * tvz[1];
+    eo_apply<N, 1>(tb.Sb, r, v);
     lds_write_line<N, N, 2>(Q, l, v);
   }
   __syncthreads();
   // ----------------------------------------------------------------- Sy
   if (act) {
+    const auto &tb = ctab(a.dt);
     T u[N], v[N];
     lds_read_line<N, N, 1>(Q, l, u);
-    eo_apply<N, 1>(ctab(a.dt).Sb, u, v);
+    eo_apply<N, 1>(tb.Sb, u, v);
     lds_write_line<N, N, 1>(Q, l, v);
   }
   __syncthreads();
   // ----------------------------------------------------------------- Sx
   if (act) {
+    const auto &tb = ctab(a.dt);
     T u[N], o[N];
     lds_read_line<N, N, 0>(Q, l, u);
-    eo_apply<N, 1>(ctab(a.dt).Sb, u, o);
+    eo_apply<N, 1>(tb.Sb, u, o);
     if constexpr (DIAG) {
       if (unit / N == l) a.dst[cell * N3 + unit] = o[unit % N];
     } else {
