@@ -38,8 +38,9 @@ namespace sfk {
 template <int N>
 constexpr int gen_cells_per_block()
 {
-  // default: 4 N^3 + 12 N^2 values of LDS per cell
-  return N <= 3 ? 256 / (N * N) : (N == 4 ? 12 : (N == 5 ? 8 : (N == 6 ? 5 : (N == 7 ? 3 : (N == 8 ? 2 : 1)))));
+  // default (4 N^3 + 12 N^2 values of LDS per cell); N = 6: 3 cells (108
+  // threads in two waves) measured fastest on MI355X for the 32^3 workload
+  return N <= 3 ? 256 / (N * N) : (N == 4 ? 12 : (N == 5 ? 8 : (N == 6 ? 3 : (N == 7 ? 3 : (N == 8 ? 2 : 1)))));
 }
 constexpr int kGenMaxThreads = 256;
 template <int N, typename T, bool DIAG>

@@ -30,7 +30,7 @@ int default_cb(const sfk_op *op)
 {
   const int n = op->d.degree + 1;
   if (op->d.kind == SFK_DG_SIPG && op->d.geometry == SFK_GEOM_PER_Q) {
-    static const int tab[10] = {0, 0, 64, 28, 12, 8, 5, 3, 2, 1};
+    static const int tab[10] = {0, 0, 64, 28, 12, 8, 3, 3, 2, 1};
     return tab[n];
   }
   return kMaxThreads / (n * n) > 0 ? kMaxThreads / (n * n) : 1;

@@ -87,7 +87,7 @@ def test_setters_validate_sizes():
 
 
 @pytest.mark.parametrize("degree,kind,geom,default_cb", [
-    (5, sfk.DG_SIPG, sfk.GEOM_PER_Q, 5), (5, sfk.DG_SIPG, sfk.GEOM_CARTESIAN, 7),
+    (5, sfk.DG_SIPG, sfk.GEOM_PER_Q, 3), (5, sfk.DG_SIPG, sfk.GEOM_CARTESIAN, 7),
     (4, sfk.CONT_LAPLACE, sfk.GEOM_CARTESIAN, 10), (8, sfk.CELL_LAPLACE, sfk.GEOM_CARTESIAN, 3),
     (3, sfk.DG_ADVECT, sfk.GEOM_CARTESIAN, 16)])
 def test_schedule_defaults_and_validation(degree, kind, geom, default_cb):

@@ -101,7 +101,7 @@ def test_schedule_validation(cuda):
         with pytest.raises(sfk.SfkError):
             op.set_schedule(*bad)
     op.set_schedule(0, 0, -1)
-    assert op.schedule[0] == 5  # the degree's default
+    assert op.schedule[0] == 3  # the degree's default
 
 
 def test_tune_picks_valid_schedule(cuda):
