@@ -123,6 +123,23 @@ struct Solver {
     return SFK_OK;
   }
 
+  // read back `count` sums that a fused kernel left in dres
+  sfk_status results(double *out, int count)
+  {
+    hipError_t e = hipMemcpyAsync(out, dres, count * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error(std::string("solver: ") + hipGetErrorString(e));
+      return SFK_ERR_HIP;
+    }
+    for (int i = 0; i < count; ++i)
+      if (!std::isfinite(out[i])) {
+        set_error("solver: non-finite inner product");
+        return SFK_ERR_NONFINITE;
+      }
+    return SFK_OK;
+  }
+
   // (x . y, x . w) in one pass, FP64 vectors
   sfk_status dot2(const void *x, const void *y, const void *w, double &xy, double &xw)
   {
@@ -283,20 +300,27 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   double res = S.st.initial_residual;
   int iters = 0;
 
-  // preconditioner z = P(r) (FP64 in and out): the Chebyshev polynomial on
-  // the smoother operator, in the smoother's precision; *z64 receives the
-  // FP64 buffer that holds the result
-  auto precond = [&](const void *r64, void **z64) -> sfk_status {
+  // preconditioner z = P(r) (FP64 out): the Chebyshev polynomial on the
+  // smoother operator, in the smoother's precision.  FP32: rs already holds
+  // (float) r and the result is widened into e64 together with the inner
+  // products z.r and z.q (zr_zq != nullptr).  *z64 receives the FP64 buffer
+  // that holds z.
+  auto precond = [&](const void *r64, const void *qv, void **z64, double *zr_zq) -> sfk_status {
     if (f32) {
       void *z = zs.p, *spare = qs.p;
-      SOLVER_HIP(vec_copy_convert(true, r64, rs.p, n, s));
       SOLVER_ST(cheby(rs.p, &z, &spare));
-      SOLVER_HIP(vec_copy_convert(false, z, e64.p, n, s));
+      if (zr_zq) {
+        SOLVER_HIP(vec_widen_dot2(n, z, e64.p, r64, qv, S.dres, S.partials, kPartials, s));
+        SOLVER_ST(S.results(zr_zq, 2));
+      } else {
+        SOLVER_HIP(vec_copy_convert(false, z, e64.p, n, s));
+      }
       *z64 = e64.p;
     } else {
       void *z = e64.p, *spare = zs.p;
       SOLVER_ST(cheby(r64, &z, &spare));
       *z64 = z;
+      if (zr_zq) SOLVER_ST(S.dot2(z, r64, qv, zr_zq[0], zr_zq[1]));
     }
     return SFK_OK;
   };
@@ -308,7 +332,9 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   const double target = rel_tol * bnorm;
   if (res > target) {
     void *z64 = nullptr, *pv = w.p, *qv = lt.p;
-    SOLVER_ST(precond(r.p, &z64));
+    void *rs32 = f32 ? rs.p : nullptr; // the CG update also writes (float) r
+    if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rs.p, n, s));
+    SOLVER_ST(precond(r.p, nullptr, &z64, nullptr));
     SOLVER_HIP(hipMemcpyAsync(pv, z64, n8, hipMemcpyDeviceToDevice, s));
     double rz;
     SOLVER_ST(S.dot(false, r.p, z64, rz));
@@ -322,10 +348,10 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         return SFK_ERR_NONFINITE;
       }
       const double alpha = rz / pq;
-      SOLVER_HIP(vec_axpby(false, n, alpha, pv, 1., x, s));
-      SOLVER_HIP(vec_axpby(false, n, -alpha, qv, 1., r.p, s));
+      // x += alpha p, r -= alpha q, (rs = (float) r), r.r in one pass
+      SOLVER_HIP(vec_cg_update(n, alpha, pv, qv, x, r.p, rs32, S.dres, S.partials, kPartials, s));
+      SOLVER_ST(S.results(&rr, 1));
       ++iters;
-      SOLVER_ST(S.dot(false, r.p, r.p, rr));
       res = std::sqrt(rr);
       if (res <= target) {
         // the recursively updated residual drifts from b - A x: confirm
@@ -336,16 +362,16 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         SOLVER_ST(S.dot(false, r.p, r.p, rr));
         res = std::sqrt(rr);
         if (res <= target) break;
-        SOLVER_ST(precond(r.p, &z64));
+        if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rs.p, n, s));
+        SOLVER_ST(precond(r.p, nullptr, &z64, nullptr));
         SOLVER_HIP(hipMemcpyAsync(pv, z64, n8, hipMemcpyDeviceToDevice, s));
         SOLVER_ST(S.dot(false, r.p, z64, rz));
         continue;
       }
-      SOLVER_ST(precond(r.p, &z64));
-      double zr, zq;
-      SOLVER_ST(S.dot2(z64, r.p, qv, zr, zq));
-      const double beta_cg = -alpha * zq / rz;
-      rz = zr;
+      double zrq[2];
+      SOLVER_ST(precond(r.p, qv, &z64, zrq));
+      const double beta_cg = -alpha * zrq[1] / rz;
+      rz = zrq[0];
       SOLVER_HIP(vec_axpby(false, n, 1., z64, beta_cg, pv, s));
     }
   }

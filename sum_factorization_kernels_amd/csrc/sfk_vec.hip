@@ -90,6 +90,58 @@ __global__ void __launch_bounds__(kThreads) dot2_partial_kernel(const double *x,
   }
 }
 
+// CG update x += alpha p, r -= alpha q (FP64), optionally rs = (float) r,
+// with per-block partial sums of r.r
+__global__ void __launch_bounds__(kThreads) cg_update_kernel(long long n, double alpha, const double *p,
+                                                             const double *q, double *x, double *r, float *rs,
+                                                             double *partials)
+{
+  __shared__ double red[kThreads];
+  double acc = 0.;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    if (rs) rs[i] = float(ri);
+    acc += ri * ri;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// z = (double) zs with per-block partial sums of z.r and z.q
+__global__ void __launch_bounds__(kThreads) widen_dot2_kernel(long long n, const float *zs, double *z,
+                                                              const double *r, const double *q, double *partials)
+{
+  __shared__ double red[2][kThreads];
+  double a = 0., b = 0.;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double zi = double(zs[i]);
+    z[i] = zi;
+    a += zi * r[i];
+    b += zi * q[i];
+  }
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = red[0][0];
+    partials[gridDim.x + blockIdx.x] = red[1][0];
+  }
+}
+
 template <typename T>
 __global__ void axpby_kernel(long long n, T a, const T *x, T b, T *y)
 {
@@ -192,6 +244,30 @@ hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, do
   if ((int)g > n_partials) g = n_partials;
   hipLaunchKernelGGL(dot2_partial_kernel, dim3(g), dim3(kThreads), 0, s, (const double *)x, (const double *)y,
                      (const double *)w, n, partials);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials + g, (int)g,
+                     result_dev + 1);
+  return hipGetLastError();
+}
+
+hipError_t vec_cg_update(long long n, double alpha, const void *p, const void *q, void *x, void *r, void *rs,
+                         double *result_dev, double *partials, int n_partials, hipStream_t s)
+{
+  unsigned g = grid_for(n, 8);
+  if ((int)g > n_partials) g = n_partials;
+  hipLaunchKernelGGL(cg_update_kernel, dim3(g), dim3(kThreads), 0, s, n, alpha, (const double *)p, (const double *)q,
+                     (double *)x, (double *)r, (float *)rs, partials);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
+  return hipGetLastError();
+}
+
+hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, const void *q, double *result_dev,
+                          double *partials, int n_partials, hipStream_t s)
+{
+  unsigned g = grid_for(n, 8);
+  if ((int)g > n_partials) g = n_partials;
+  hipLaunchKernelGGL(widen_dot2_kernel, dim3(g), dim3(kThreads), 0, s, n, (const float *)zs, (double *)z,
+                     (const double *)r, (const double *)q, partials);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials + g, (int)g,
                      result_dev + 1);
