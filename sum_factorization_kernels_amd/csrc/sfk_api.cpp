@@ -431,31 +431,11 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
   }
   if (d.kind == SFK_CONT_LAPLACE || d.kind == SFK_CELL_LAPLACE) {
     SFK_REQUIRE(d.n_rhs == 1, SFK_ERR_UNSUPPORTED, "sfk_cheby_step: n_rhs must be 1");
-    SFK_REQUIRE(!(d.flags & SFK_FLAG_VARIABLE_COEF) || op->coef, SFK_ERR_NOT_READY, "coefficient not set");
     sfk_status st = ensure_scratch(op, (size_t)op->nvec * elem_size(op));
     if (st != SFK_OK) return st;
-    if (d.kind == SFK_CELL_LAPLACE) {
-      st = sfk_apply(op, sol_old, op->scratch, stream);
-      if (st != SFK_OK) return st;
-      SFK_CHECK_HIP(vec_cheby_update(fp32, op->nvec, op->scratch, rhs, sol_old, diag, sol_new, tmp, c_np, c_tm, s));
-      return SFK_OK;
-    }
-    // continuous: the colored kernel applies the update as each node's last
-    // contribution arrives (scratch: the node accumulator)
-    CellLaunch L;
-    SFK_ENSURE_TABLES(op);
-    fill_cell_launch(op, L);
-    L.mode = 1;
-    L.cheby = true;
-    L.src = sol_old;
-    L.dst = sol_new;
-    L.rhs = rhs;
-    L.diag = diag;
-    L.tm = tmp;
-    L.c_np = c_np;
-    L.c_tm = c_tm;
-    L.acc = op->scratch;
-    SFK_CHECK_HIP(launch_cell(L, s));
+    st = sfk_apply(op, sol_old, op->scratch, stream);
+    if (st != SFK_OK) return st;
+    SFK_CHECK_HIP(vec_cheby_update(fp32, op->nvec, op->scratch, rhs, sol_old, diag, sol_new, tmp, c_np, c_tm, s));
     return SFK_OK;
   }
   set_error("sfk_cheby_step: not supported for advection");

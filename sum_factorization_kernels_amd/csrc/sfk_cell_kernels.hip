@@ -54,7 +54,6 @@ struct CellArgs {
   T *dst;
   const T *rhs, *diag; // Chebyshev
   T *tm;
-  T *acc;              // continuous Chebyshev: partial sums of nodes shared by colours
   T c_np, c_tm;
   const T *coef;   // per cell q-point coefficient (optional)
   long long ncells;
@@ -139,62 +138,35 @@ __device__ __forceinline__ void grad_phase(const CellArgs<N, T> &a, const T *Ace
 // later ones add, which makes a zero fill and atomics unnecessary.  Inside a
 // segment the first x-plane of cell c+1 (left in LDS by its threads) is
 // added to the last x-plane of cell c.
-template <int N, typename T, bool CHEBY>
+template <int N, typename T>
 __device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&o)[N], const T *sB, int c, int l,
-                                                   int cx, int cy, int cz, int seg, int cx0, int cx_end, T *dst,
-                                                   const T *src)
+                                                   int cx, int cy, int cz, int seg, int cx0, int cx_end, T *dst)
 {
   constexpr int p = N - 1, N3 = N * N * N;
   const int j = l % N, k = l / N;
   if (cx + 1 < cx_end) o[p] += sB[(c + 1) * N3 + l];
   const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
   const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;
-  // smallest / largest colour of the units covering rows [ylo,yhi] x
-  // [zlo,zhi] x segments [slo,shi] (each range has one or two entries, and a
-  // range of two holds both parities)
-  auto color_of = [&](int slo, int shi, int two) {
-    const int py = ylo == yhi ? (ylo & 1) : two, pz = zlo == zhi ? (zlo & 1) : two, ps = slo == shi ? (slo & 1) : two;
+  // smallest colour of the units covering rows [ylo,yhi] x [zlo,zhi] x
+  // segments [slo,shi] (each range has one or two entries)
+  auto min_color = [&](int slo, int shi) {
+    const int py = ylo == yhi ? (ylo & 1) : 0, pz = zlo == zhi ? (zlo & 1) : 0, ps = slo == shi ? (slo & 1) : 0;
     return py + 2 * pz + 4 * ps;
   };
-  const int m_mid = color_of(seg, seg, 0), last_mid = color_of(seg, seg, 1);
+  const int m_mid = min_color(seg, seg);
   const int Y = cy * p + j, Z = cz * p + k;
   const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
   const long long g0 = ((long long)Z * a.gy + Y) * a.gx + (long long)cx * p;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     if (i == 0 && cx > cx0) continue; // added into the previous cell's plane
-    int m = m_mid, last = last_mid;
-    if (i == 0 && seg > 0) {
-      m = color_of(seg - 1, seg, 0);
-      last = color_of(seg - 1, seg, 1);
-    }
-    if (i == p && cx + 1 == cx_end && seg + 1 < a.nseg) {
-      m = color_of(seg, seg + 1, 0);
-      last = color_of(seg, seg + 1, 1);
-    }
+    int m = m_mid;
+    if (i == 0 && seg > 0) m = min_color(seg - 1, seg);
+    if (i == p && cx + 1 == cx_end && seg + 1 < a.nseg) m = min_color(seg, seg + 1);
     const int X = cx * p + i;
     const bool bd = byz || X == 0 || X == a.gx - 1;
-    const long long g = g0 + i;
-    if constexpr (!CHEBY) {
-      if (m == a.color) dst[g] = bd ? T(0) : o[i];
-      else if (!bd) dst[g] += o[i];
-    } else {
-      // earlier colours accumulate the node's sum in acc; the last colour
-      // completes A u and applies the Chebyshev update of
-      // evaluation_dg_laplacian.h:485-502 (Dirichlet nodes: A u = 0)
-      if (last != a.color) {
-        if (!bd) {
-          if (m == a.color) a.acc[g] = o[i];
-          else a.acc[g] += o[i];
-        }
-        continue;
-      }
-      const T au = bd ? T(0) : (m == a.color ? o[i] : a.acc[g] + o[i]);
-      const T res = au - a.rhs[g];
-      const T tmv = (a.c_tm != T(0) ? a.c_tm * a.tm[g] : T(0)) + a.c_np * a.diag[g] * res;
-      a.tm[g] = tmv;
-      dst[g] = src[g] - tmv;
-    }
+    if (m == a.color) dst[g0 + i] = bd ? T(0) : o[i];
+    else if (!bd) dst[g0 + i] += o[i];
   }
 }
 
@@ -393,7 +365,7 @@ cell_kernel(const CellArgs<N, T> a)
     if constexpr (MODE == Mode::Cont) {
       if (a.color >= 0) {
         __syncthreads();
-        if (act) cont_store_colored<N, T, CHEBY>(a, o, sB, c, l, ccx, ccy, ccz, seg, cx0, cx_end, dst, src);
+        if (act) cont_store_colored<N, T>(a, o, sB, c, l, ccx, ccy, ccz, seg, cx0, cx_end, dst);
       } else if (act) {
         const int Y = ccy * p + l % N, Z = ccz * p + l / N;
         const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
@@ -704,22 +676,6 @@ static hipError_t launch_cell_n(const CellLaunch &L, hipStream_t stream)
   } else if (L.mode == 1) {
     auto kern = L.coef ? cell_kernel<N, T, Mode::Cont, false, false, true>
                        : cell_kernel<N, T, Mode::Cont, false, false, false>;
-    if (L.cheby) {
-      // fused Chebyshev step: colored schedule only (the last colour to
-      // touch a node applies the update); otherwise matvec into acc and the
-      // separate vector update
-      const bool fp32 = std::is_same<T, float>::value;
-      if (!cont_colored(L, CB)) {
-        CellLaunch M = L;
-        M.cheby = false;
-        M.dst = L.acc;
-        const hipError_t e = launch_cell_n<N, T>(M, stream);
-        if (e != hipSuccess) return e;
-        return vec_cheby_update(fp32, L.vstride, L.acc, L.rhs, L.src, L.diag, L.dst, L.tm, L.c_np, L.c_tm, stream);
-      }
-      kern = L.coef ? cell_kernel<N, T, Mode::Cont, true, false, true> : cell_kernel<N, T, Mode::Cont, true, false, false>;
-      a.acc = (T *)L.acc;
-    }
     if (!cont_colored(L, CB)) {
       a.color = -1;
       const hipError_t e = vec_fill(std::is_same<T, float>::value, L.dst, L.vstride * L.nrhs, 0., stream);

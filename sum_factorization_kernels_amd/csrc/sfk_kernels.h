@@ -36,7 +36,6 @@ struct CellLaunch {
   double inv_jac[3] = {1., 1., 1.};
   int nrhs = 1;
   long long vstride = 0;
-  void *acc = nullptr;     // continuous Chebyshev step: node accumulator (vector size)
   bool load = false;       // launch_cell_diagonal: load vector of a constant instead
   double load_value = 0.;
   int cont_scatter = -1;   // continuous: -1 automatic, 0 zero fill + atomics, 1 colored
