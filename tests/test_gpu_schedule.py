@@ -114,3 +114,27 @@ def test_tune_picks_valid_schedule(cuda):
     assert op.schedule == sched
     src = np.random.default_rng(4300).uniform(-1, 1, op.vector_size)
     assert rel_err(to_np(op.apply(to_dev(src))), ob.cont_laplacian(degree, nc, ij, src)) < TOL_FP64
+
+
+@pytest.mark.parametrize("degree", [1, 3, 6])
+@pytest.mark.parametrize("scatter", [0, 1])
+@pytest.mark.parametrize("c_tm", [0.0, 0.37])
+def test_cont_chebyshev_step(cuda, degree, scatter, c_tm):
+    """continuous Chebyshev step under both scatter schedules against
+    tm' = c_tm tm + c_np diag (A u - rhs), u' = u - tm' with A u from the oracle"""
+    nc = (5, 3, 4)
+    ij = (5.0, 3.0, 4.0)
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    op.set_schedule(2, 1, scatter)
+    N = op.vector_size
+    rng = np.random.default_rng(4400 + degree)
+    u, rhs = rng.uniform(-1, 1, N), rng.standard_normal(N)
+    diag, tm = rng.uniform(0.5, 1.5, N), rng.standard_normal(N)
+    if c_tm == 0.0:
+        tm[:] = np.nan  # not read when c_tm == 0
+    d_new, d_tm = to_dev(np.zeros(N)), to_dev(tm)
+    op.cheby_step(to_dev(rhs), to_dev(u), to_dev(diag), d_new, d_tm, 0.8, c_tm)
+    au = ob.cont_laplacian(degree, nc, ij, u)
+    r_tm = (c_tm * tm if c_tm != 0.0 else 0.0) + 0.8 * diag * (au - rhs)
+    assert rel_err(to_np(d_tm), r_tm) < TOL_FP64
+    assert rel_err(to_np(d_new), u - r_tm) < TOL_FP64
