@@ -46,6 +46,17 @@ def sine_metric(degree, nc, amp):
         J += grad[..., None, :]  # every row gets grad s (x_i = xhat_i + s)
         return J
 
+    def det_inv(J):
+        # closed-form 3x3 determinant and inverse (adjugate / det)
+        a = [[J[..., i, j] for j in range(3)] for i in range(3)]
+        adj = np.empty_like(J)  # adj[j, i] = cofactor (i, j)
+        for i in range(3):
+            for j in range(3):
+                i1, i2, j1, j2 = (i + 1) % 3, (i + 2) % 3, (j + 1) % 3, (j + 2) % 3
+                adj[..., j, i] = a[i1][j1] * a[i2][j2] - a[i1][j2] * a[i2][j1]
+        det = a[0][0] * adj[..., 0, 0] + a[0][1] * adj[..., 1, 0] + a[0][2] * adj[..., 2, 0]
+        return det, adj / det[..., None, None]
+
     ncell = int(np.prod(nc))
     iz, iy, ix = np.meshgrid(np.arange(nc[2]), np.arange(nc[1]), np.arange(nc[0]), indexing="ij")
     cells = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=-1).astype(np.float64)  # (ncell, 3)
@@ -55,8 +66,7 @@ def sine_metric(degree, nc, amp):
     wq = np.prod(wg[qi], axis=-1)
     xh = (cells[:, None, :] + xi[None, :, :]) * h  # (ncell, n3, 3)
     J = jac(xh) * h  # dx/dxi
-    det = np.linalg.det(J)
-    Ji = np.linalg.inv(J)
+    det, Ji = det_inv(J)
     G = (det * wq)[..., None, None] * np.einsum("cqij,cqkj->cqik", Ji, Ji)
     cm = np.stack([G[..., 0, 0], G[..., 1, 1], G[..., 2, 2], G[..., 0, 1], G[..., 0, 2], G[..., 1, 2]], axis=1)
     cm = np.ascontiguousarray(cm.reshape(ncell, 6, n ** 3)).ravel()
@@ -80,9 +90,8 @@ def sine_metric(degree, nc, amp):
         up[:, d] = (up[:, d] + 1) % nc[d]
         JL = jac((cells[:, None, :] + xiL[None]) * h) * h
         JU = jac((up[:, None, :] + xiU[None]) * h) * h
-        detL = np.linalg.det(JL)
-        JLi = np.linalg.inv(JL)
-        JUi = np.linalg.inv(JU)
+        detL, JLi = det_inv(JL)
+        JUi = det_inv(JU)[1]
         nvec = np.einsum("cqji,j->cqi", JLi, e)  # J^{-T} e_d
         nrm = np.linalg.norm(nvec, axis=-1)
         area = detL * nrm
