@@ -37,6 +37,7 @@
 #include "evaluation_cell_laplacian.h"
 #include "evaluation_dg_laplacian.h"
 #include "evaluation_dg_advect.h"
+#include "precomputed_dg_laplacian.h"
 #undef private
 
 typedef VectorizedArray<double> VA;
@@ -318,6 +319,37 @@ static int advect_kat_dispatch(int p, const unsigned *nc, int n_steps, double *e
   return 0;
 }
 
+// --------------------------------------------- precomputed DG Laplacian
+template <int degree>
+static int precomp_dispatch(int p, const unsigned *nc, const double *src, double *dst, double *mass_eo,
+                            double *lap_eo, double *outer)
+{
+  if (p != degree) {
+    if constexpr (degree < 10)
+      return precomp_dispatch<degree + 1>(p, nc, src, dst, mass_eo, lap_eo, outer);
+    else
+      return -1;
+  }
+  auto e = std::make_unique<PrecomputedDGLaplacian<3, degree, double>>();
+  // block sizes of legacy/src/test_dg_precomputed.cc
+  e->blx = std::max(20 / (degree + 1), 3);
+  e->bly = std::max(2, 20 / (degree + 1));
+  e->blz = std::max(2, 20 / (degree + 1));
+  unsigned n_cells[3] = {nc[0], nc[1], nc[2]};
+  e->initialize(n_cells);
+  if (src) {
+    load_scalar(e->input_array, src);
+    e->matrix_vector_product();
+    store_scalar(e->output_array, dst);
+  }
+  const int n = degree + 1, h = (n + 1) / 2;
+  copy_va_lane0(e->mass_1d_eo, mass_eo, n * h);
+  copy_va_lane0(e->laplace_1d_eo, lap_eo, n * h);
+  outer[0] = e->value_outer_1;
+  outer[1] = e->value_outer_2;
+  return 0;
+}
+
 extern "C" {
 
 __attribute__((visibility("default"))) int ref_n_lanes() { return L; }
@@ -376,6 +408,17 @@ __attribute__((visibility("default"))) int ref_dg_advect(int degree, const unsig
 {
   return advect_dispatch<1>(degree, n_cells, inv_jac, mode, src, upd, tm_in, c_np, c_tm, out_np,
                             out_tm);
+}
+
+// PrecomputedDGLaplacian (legacy/include/precomputed_dg_laplacian.h): dst =
+// matrix_vector_product() of src (lane-interleaved, n_cells[0] counts
+// cells); src = NULL only fills the tables.  mass_eo, lap_eo: n*ceil(n/2)
+// even-odd tables; outer = {value_outer_1, value_outer_2}.
+__attribute__((visibility("default"))) int ref_dg_precomputed(int degree, const unsigned *n_cells, const double *src,
+                                                              double *dst, double *mass_eo, double *lap_eo,
+                                                              double *outer)
+{
+  return precomp_dispatch<1>(degree, n_cells, src, dst, mass_eo, lap_eo, outer);
 }
 
 __attribute__((visibility("default"))) int ref_advect_kat(int degree, const unsigned *n_cells, int n_steps, double *err_deriv,

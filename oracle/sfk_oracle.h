@@ -134,6 +134,14 @@ void sfko_dg_advect_var_transpose(int degree, const int nc[3], const double inv_
 void sfko_dg_advect_velocity_grad(int degree, const int nc[3], const double inv_jac[3], const double *face_vel,
                                   const double *u, const double *lambda, double *grad_cell, double *grad_face);
 
+/* ---- PrecomputedDGLaplacian (precomputed_dg_laplacian.h:96-377) ----
+ * 1D mass / Laplace(+penalty) matrices [i*n+j] and the neighbour coupling
+ * v = {v1, v2}; ref_coupling selects the reference's (vanishing) coupling
+ * constants instead of the SIPG ones. */
+void sfko_precomp_1d(int degree, int ref_coupling, double *mass, double *lap, double *v);
+void sfko_dg_precomputed(int degree, const int nc[3], const double inv_jac[3], int ref_coupling,
+                         const double *src, double *dst);
+
 /* ---- layout converters (vectorization.h lane interleaving,
  * evaluation_dg_laplacian.h:162-166): interleaved index
  *   ((iz*ny+iy)*(nx/L) + ixb)*L*n^3 + dof*L + lane,  ix = ixb*L + lane. */

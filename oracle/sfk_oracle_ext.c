@@ -456,3 +456,113 @@ void sfko_dg_advect_velocity_grad(int degree, const int nc[3], const double inv_
   }
   free(mu);
 }
+
+/* ------------------------------------------------------------------ */
+/* PrecomputedDGLaplacian, legacy/include/precomputed_dg_laplacian.h    */
+/* ------------------------------------------------------------------ */
+
+/* 1D matrices of fill_shape_values (:305-349) on the unit interval,
+ * Hermite-like basis, Gauss quadrature with n = p+1 points, penalty
+ * p(p+1) (:310):
+ *   mass[i][j] = sum_q phi_i phi_j w_q,
+ *   lap[i][j]  = sum_q phi_i' phi_j' w_q + the cell's own interior-penalty
+ *                face terms at x = 0 and x = 1 (:322-330).
+ * Neighbour coupling v[0] = v1, v[1] = v2 (rows 0 and 1 of a cell against
+ * the left neighbour's dofs n-1 and n-2, :139-148):
+ *   ref_coupling != 0: the reference's expressions (:331-340), which
+ *     evaluate phi_0 at x = 1 and phi_{n-1}, phi_{n-2} at x = 0, where
+ *     these functions vanish (both constants are zero);
+ *   ref_coupling == 0: the SIPG face term between the test function phi_0
+ *     of a cell (its x = 0) and the trial function phi_j of its left
+ *     neighbour (its x = 1):
+ *       C_j = 1/2 phi_j'(1) phi_0(0) - 1/2 phi_j(1) phi_0'(0) - sigma phi_0(0) phi_j(1),
+ *     v1 = C_{n-1}, v2 = C_{n-2}.
+ * Pinned by a brute-force assembly of the 1D SIPG form with an
+ * independent quadrature and by A 1 = 0 (tests/test_oracle_precomputed.py). */
+void sfko_precomp_1d(int degree, int ref_coupling, double *mass, double *lap, double *v)
+{
+  const int n = degree + 1;
+  double x[MAXN], w[MAXN], val[MAXN][MAXN], der[MAXN][MAXN];
+  double v0[MAXN], v1[MAXN], d0[MAXN], d1[MAXN];
+  sfko_gauss_points(n, x);
+  sfko_gauss_weights(n, w);
+  for (int i = 0; i < n; ++i) {
+    for (int q = 0; q < n; ++q) {
+      val[i][q] = sfko_hermite_value(degree, i, x[q]);
+      der[i][q] = sfko_hermite_derivative(degree, i, x[q]);
+    }
+    v0[i] = sfko_hermite_value(degree, i, 0.);
+    v1[i] = sfko_hermite_value(degree, i, 1.);
+    d0[i] = sfko_hermite_derivative(degree, i, 0.);
+    d1[i] = sfko_hermite_derivative(degree, i, 1.);
+  }
+  const double sigma = (double)(n - 1) * n;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double sm = 0, sl = 0;
+      for (int q = 0; q < n; ++q) {
+        sm += val[i][q] * val[j][q] * w[q];
+        sl += der[i][q] * der[j][q] * w[q];
+      }
+      mass[i * n + j] = sm;
+      sl += v0[i] * v0[j] * sigma + 0.5 * d0[i] * v0[j] + 0.5 * d0[j] * v0[i];
+      sl += v1[i] * v1[j] * sigma - 0.5 * d1[i] * v1[j] - 0.5 * d1[j] * v1[i];
+      lap[i * n + j] = sl;
+    }
+  const int a = n - 1, b = n >= 2 ? n - 2 : 0;
+  if (ref_coupling) {
+    v[0] = -v1[0] * v0[a] * sigma + 0.5 * d1[0] * v0[a] - 0.5 * d0[a] * v1[0];
+    v[1] = -v1[0] * v0[b] * sigma + 0.5 * d1[0] * v0[b] - 0.5 * d0[b] * v1[0];
+  } else {
+    v[0] = 0.5 * d1[a] * v0[0] - 0.5 * v1[a] * d0[0] - sigma * v0[0] * v1[a];
+    v[1] = 0.5 * d1[b] * v0[0] - 0.5 * v1[b] * d0[0] - sigma * v0[0] * v1[b];
+  }
+}
+
+/* dst = A src of PrecomputedDGLaplacian::do_inner_loop (:97-278), periodic,
+ * canonical layout, on Cartesian cells of size 1/inv_jac (the reference:
+ * unit cells, inv_jac = 1):
+ *   A = f_x M (x) M (x) L~_x + f_y M (x) L~_y (x) M + f_z L~_z (x) M (x) M,
+ * L~_d = lap plus the neighbour coupling along d, f_d = inv_jac_d^2 /
+ * (inv_jac_x inv_jac_y inv_jac_z) (f_d = 1 on unit cells). */
+void sfko_dg_precomputed(int degree, const int nc[3], const double inv_jac[3], int ref_coupling,
+                         const double *src, double *dst)
+{
+  const int n = degree + 1, n2 = n * n, n3 = n2 * n;
+  double M[MAXN * MAXN], Lm[MAXN * MAXN], v[2];
+  sfko_precomp_1d(degree, ref_coupling, M, Lm, v);
+  const double det = inv_jac[0] * inv_jac[1] * inv_jac[2];
+  double f[3];
+  for (int d = 0; d < 3; ++d) f[d] = inv_jac[d] * inv_jac[d] / det;
+  const long ncell = (long)nc[0] * nc[1] * nc[2];
+  double a[3][MAXN * MAXN * MAXN], t1[MAXN * MAXN * MAXN], t2[MAXN * MAXN * MAXN];
+  for (long c = 0; c < ncell; ++c) {
+    const double *u = src + c * n3;
+    for (int d = 0; d < 3; ++d) {
+      apply_dir(n, 3, d, Lm, 0, u, a[d]);
+      const double *ul = src + neighbor(nc, c, d, 0) * n3, *uh = src + neighbor(nc, c, d, 1) * n3;
+      for (int fa = 0; fa < n2; ++fa) {
+#define VI(layer) vol_index(n, d, (layer), fa)
+        a[d][VI(0)] += v[0] * ul[VI(n - 1)] + v[1] * ul[VI(n >= 2 ? n - 2 : 0)];
+        a[d][VI(1)] += v[1] * ul[VI(n - 1)];
+        a[d][VI(n - 1)] += v[0] * uh[VI(0)] + v[1] * uh[VI(1)];
+        a[d][VI(n - 2)] += v[1] * uh[VI(0)];
+#undef VI
+      }
+      for (int i = 0; i < n3; ++i) a[d][i] *= f[d];
+    }
+    double *out = dst + c * n3;
+    /* M_z M_y a_x */
+    apply_dir(n, 3, 1, M, 0, a[0], t1);
+    apply_dir(n, 3, 2, M, 0, t1, t2);
+    for (int i = 0; i < n3; ++i) out[i] = t2[i];
+    /* M_z M_x a_y */
+    apply_dir(n, 3, 0, M, 0, a[1], t1);
+    apply_dir(n, 3, 2, M, 0, t1, t2);
+    for (int i = 0; i < n3; ++i) out[i] += t2[i];
+    /* M_y M_x a_z */
+    apply_dir(n, 3, 0, M, 0, a[2], t1);
+    apply_dir(n, 3, 1, M, 0, t1, t2);
+    for (int i = 0; i < n3; ++i) out[i] += t2[i];
+  }
+}

@@ -158,3 +158,61 @@ def dense_from_apply(apply, N):
         e[i] = 1.0
         A[:, i] = apply(e)
     return A
+
+
+def dg_precomputed_dense(degree, nc, inv_jac):
+    """Dense periodic DG-SIPG matrix of the Hermite-like basis on Cartesian
+    cells of size h_d = 1/inv_jac_d: volume term grad u . grad v and, on every
+    face with unit normal e_d from the lower cell (-) to the upper cell (+),
+        -{du/dn}[v] - [u]{dv/dn} + sigma [u][v],
+    [w] = w- - w+, {dw/dn} = (dw-/dx_d + dw+/dx_d) / 2, sigma = p(p+1)/h_d
+    (the penalty of precomputed_dg_laplacian.h:310 on cells of size h_d).
+    Quadrature: p+3 Gauss points per direction (more than the operator's)."""
+    n = degree + 1
+    n3 = n ** 3
+    h = 1.0 / np.asarray(inv_jac, dtype=np.float64)
+    xq, wq = np.polynomial.legendre.leggauss(n + 2)
+    xq, wq = 0.5 * (xq + 1.0), 0.5 * wq
+    v, d = basis_1d(degree, 0, xq)          # (n, nq)
+    v01, d01 = basis_1d(degree, 0, [0.0, 1.0])  # (n, 2)
+    ii = dof_triples(n)                      # (3, n3)
+    ncell = int(np.prod(nc))
+    A = np.zeros((ncell * n3, ncell * n3))
+    det = np.prod(h)
+    # element stiffness (identical for every cell)
+    K = np.zeros((n3, n3))
+    for dd in range(3):
+        # 1D factors: derivative in dd, values in the others
+        f = []
+        for e in range(3):
+            a = d if e == dd else v
+            f.append((a[:, None, :] * a[None, :, :] * wq).sum(axis=2))  # (n, n)
+        K += det / h[dd] ** 2 * f[0][ii[0][:, None], ii[0][None, :]] * f[1][ii[1][:, None], ii[1][None, :]] \
+            * f[2][ii[2][:, None], ii[2][None, :]]
+    for c in range(ncell):
+        A[c * n3:(c + 1) * n3, c * n3:(c + 1) * n3] += K
+    # faces
+    sigma = degree * (degree + 1)
+    for c in range(ncell):
+        ix, iy, iz = c % nc[0], (c // nc[0]) % nc[1], c // (nc[0] * nc[1])
+        for dd in range(3):
+            up = [ix, iy, iz]
+            up[dd] = (up[dd] + 1) % nc[dd]
+            cp = cell_id(nc, *up)
+            t = [e for e in range(3) if e != dd]
+            # tangential mass factors (face quadrature over the two other directions)
+            tm = [(v[:, None, :] * v[None, :, :] * wq).sum(axis=2) * h[e] for e in t]  # (n, n) each
+            T = tm[0][ii[t[0]][:, None], ii[t[0]][None, :]] * tm[1][ii[t[1]][:, None], ii[t[1]][None, :]]
+            # normal-direction traces: side -: x_dd = 1, side +: x_dd = 0
+            val = {-1: v01[:, 1], 1: v01[:, 0]}
+            der = {-1: d01[:, 1] / h[dd], 1: d01[:, 0] / h[dd]}
+            cells = {-1: c, 1: cp}
+            for s in (-1, 1):      # test side
+                for r in (-1, 1):  # trial side
+                    js, jr = (1.0 if s == -1 else -1.0), (1.0 if r == -1 else -1.0)
+                    vs, ds = val[s][ii[dd]], der[s][ii[dd]]
+                    vr, dr = val[r][ii[dd]], der[r][ii[dd]]
+                    loc = (-0.5 * dr[None, :] * js * vs[:, None] - jr * vr[None, :] * 0.5 * ds[:, None]
+                           + sigma / h[dd] * jr * js * vr[None, :] * vs[:, None]) * T
+                    A[cells[s] * n3:(cells[s] + 1) * n3, cells[r] * n3:(cells[r] + 1) * n3] += loc
+    return A

@@ -13,7 +13,9 @@ nx-1, legacy/include/evaluation_dg_laplacian.h:270) are recorded in `mask`
 reference's backward symmetric 1D sweep drops the middle coefficient
 (matrix_vector_kernel.h:130-131, type 0), which is exact only when the basis
 is nodal at x = 1/2 (GLL, Hermite p >= 3 odd / p >= 4 even) and not for the
-p = 2 Hermite basis.
+p = 2 Hermite basis.  PrecomputedDGLaplacian stores every degree: its
+outputs at the rows that the reference computes from wrong addresses are
+masked by the tests that read them (tests/test_oracle_precomputed.py).
 """
 import os
 import sys
@@ -133,6 +135,17 @@ def main():
         save(f"dg_advect_p{p}.npz", seed=seed, n_cells=np.array(nc),
              inv_jac=np.array([nc[0] / 2.0, nc[1] / 2.0, nc[2] / 2.0]), velocity=np.array([1.0, 0.15, -0.05]),
              c_np=0.3, c_tm=0.2, matvec=C(mv), stage_np=C(vnp), stage_tm=C(vtm))
+
+    # PrecomputedDGLaplacian (precomputed_dg_laplacian.h), unit cells
+    for p in range(1, 9):
+        nc = (2 * L, 3, 2) if p <= 5 else (L, 2, 2)
+        n3 = (p + 1) ** 3
+        N = n3 * int(np.prod(nc))
+        seed = 50 + p
+        src = np.random.default_rng(seed).standard_normal(N)
+        out, m_eo, l_eo, outer = ob.ref_dg_precomputed(p, nc, ob.to_interleaved(p, nc, L, src))
+        save(f"dg_precomputed_p{p}.npz", seed=seed, n_cells=np.array(nc), matvec=ob.to_canonical(p, nc, L, out),
+             mass_eo=m_eo, laplace_eo=l_eo, value_outer=np.array(outer))
 
     # advection known-answer tests (evaluation_dg_advect.h:221-283)
     kat = {}

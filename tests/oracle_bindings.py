@@ -78,6 +78,7 @@ def ref():
         lib.ref_dg_advect.argtypes = [ctypes.c_int, _up, _dp, ctypes.c_int, _dp, _dp, _dp,
                                       ctypes.c_double, ctypes.c_double, _dp, _dp]
         lib.ref_advect_kat.argtypes = [ctypes.c_int, _up, ctypes.c_int, _dp, _dp, _dp, _dp]
+        lib.ref_dg_precomputed.argtypes = [ctypes.c_int, _up, _dp, _dp, _dp, _dp, _dp]
         _ref = lib
     return _ref
 
@@ -214,7 +215,34 @@ def ref_dg_advect(degree, nc, inv_jac, src, upd=None, tm=None, c_np=0.0, c_tm=0.
     return out_np, out_tm
 
 
+def ref_dg_precomputed(degree, nc, src_interleaved=None):
+    """reference PrecomputedDGLaplacian: (dst interleaved or None, mass_eo,
+    laplace_eo, (value_outer_1, value_outer_2))"""
+    n = degree + 1
+    h = (n + 1) // 2
+    ncu = (ctypes.c_uint * 3)(*nc)
+    m, lap, outer = np.zeros(n * h), np.zeros(n * h), np.zeros(2)
+    dst = None if src_interleaved is None else np.zeros_like(src_interleaved)
+    rc = ref().ref_dg_precomputed(degree, ncu, _ptr(src_interleaved), _ptr(dst), _ptr(m), _ptr(lap), _ptr(outer))
+    assert rc == 0
+    return dst, m, lap, (outer[0], outer[1])
+
+
 # ------------------------------------------------------- extended oracle
+def precomp_1d(degree, ref_coupling=False):
+    """1D mass and Laplace(+penalty) matrices [n, n] and the coupling (v1, v2)"""
+    n = degree + 1
+    m, lap, v = np.zeros(n * n), np.zeros(n * n), np.zeros(2)
+    oracle().sfko_precomp_1d(degree, int(ref_coupling), _ptr(m), _ptr(lap), _ptr(v))
+    return m.reshape(n, n), lap.reshape(n, n), (v[0], v[1])
+
+
+def dg_precomputed(degree, nc, src, inv_jac=(1.0, 1.0, 1.0), ref_coupling=False):
+    dst = np.zeros_like(src)
+    oracle().sfko_dg_precomputed(degree, _i3(nc), _d3(inv_jac), int(ref_coupling), _ptr(src), _ptr(dst))
+    return dst
+
+
 def cont_laplacian(degree, nc, inv_jac, src, coef=None):
     dst = np.zeros_like(src)
     oracle().sfko_cont_laplacian(degree, _i3(nc), _d3(inv_jac), _ptr(coef), _ptr(src), _ptr(dst))
