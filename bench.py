@@ -11,8 +11,9 @@ The same line carries the FP32 matvec, one degree-5 point-Jacobi Chebyshev
 sweep, and (unless --no-extras) the other workloads: continuous Q4 16^3
 (100 repeated applies), the mixed-precision CG-Chebyshev solve of the
 continuous Q6 Laplacian on 64^3 cells against the all-FP64 solve, DG
-advection p=3 on 48^3 (forward, adjoint, velocity gradient) and the
-variable-coefficient Q8 24^3 8-RHS apply.
+advection p=3 on 48^3 (forward, adjoint, velocity gradient), the
+variable-coefficient Q8 24^3 8-RHS apply and the precomputed-matrix DG
+Laplacian p=3 on the reference driver's 56 x 50 x 55 grid.
 
 Timing: HIP events (torch.cuda.Event) on the stream every kernel is launched
 on (torch's current stream, passed through the C ABI), inputs resident in
@@ -314,6 +315,22 @@ def main():
                                                 "coef_grad_ms": round(t5g * 1e3, 4)}
         log(f"var-coef Q8 24^3 x8 FP32: {8 * o5.vector_size / t5 / 1e9:.2f} GDoF/s")
         del o5, x5, y5
+
+        # precomputed-matrix DG Laplacian, degree 3, the reference driver's
+        # default grid (test_dg_precomputed.cc: 10M dofs -> 56 x 50 x 55 cells)
+        pc = {}
+        for name, prec, dt in (("fp64", sfk.FP64, torch.float64), ("fp32", sfk.FP32, torch.float32)):
+            o6 = sfk.Operator(sfk.DG_PRECOMP, 3, (56, 50, 55), precision=prec)
+            sched6, _ = o6.tune(repeats=5)
+            x6 = torch.randn(o6.vector_size, dtype=dt, device="cuda")
+            y6 = torch.empty_like(x6)
+            t6 = timed(lambda: o6.apply(x6, y6), 50, 5)
+            pc[name] = {"gdofs": round(o6.vector_size / t6 / 1e9, 3), "apply_us": round(t6 * 1e6, 2),
+                        "hbm_gbs": round(2 * x6.element_size() * o6.vector_size / t6 / 1e9, 1),
+                        "schedule": list(sched6)}
+            log(f"precomputed DG p3 56x50x55 {name}: {o6.vector_size / t6 / 1e9:.2f} GDoF/s")
+            del o6, x6, y6
+        extra["precomputed_dg_p3_56x50x55"] = pc
 
     cpu = None
     if not args.no_cpu and not args.only_headline:

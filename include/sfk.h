@@ -49,7 +49,10 @@ typedef enum {
   SFK_CELL_LAPLACE = 0, /* element-local Laplacian, EvaluationCellLaplacian */
   SFK_CONT_LAPLACE = 1, /* continuous Q_p Laplacian, homogeneous Dirichlet  */
   SFK_DG_SIPG = 2,      /* DG symmetric interior penalty, periodic          */
-  SFK_DG_ADVECT = 3     /* DG upwind advection M^{-1}L(u), periodic         */
+  SFK_DG_ADVECT = 3,    /* DG upwind advection M^{-1}L(u), periodic         */
+  SFK_DG_PRECOMP = 4    /* DG Laplacian from precomputed 1D matrices,
+                           PrecomputedDGLaplacian (precomputed_dg_laplacian.h
+                           :96-377): Hermite-like basis, Cartesian, periodic  */
 } sfk_op_kind;
 
 typedef enum { SFK_FP64 = 0, SFK_FP32 = 1 } sfk_precision;
@@ -61,6 +64,12 @@ typedef enum { SFK_GEOM_CARTESIAN = 0, SFK_GEOM_PER_Q = 1 } sfk_geometry;
                                         (evaluation_dg_laplacian.h:481-483, S^T) */
 #define SFK_FLAG_VARIABLE_COEF 2u    /* continuous Laplacian: -div(k grad u), k per q-point */
 #define SFK_FLAG_VARIABLE_VELOCITY 4u /* advection: velocity per cell and face q-point */
+#define SFK_FLAG_REF_PRECOMP_COUPLING 8u /* SFK_DG_PRECOMP: the reference's neighbour
+                                        coupling constants (precomputed_dg_laplacian.h
+                                        :331-340), which evaluate the basis at the
+                                        opposite ends and vanish (the operator is then
+                                        block-diagonal per cell); default: the SIPG
+                                        coupling */
 
 typedef struct sfk_op_desc {
   int kind;          /* sfk_op_kind */
@@ -129,8 +138,8 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream);
 sfk_status sfk_apply_transpose(sfk_op *op, const void *src, void *dst, void *stream);
 /* Fused Chebyshev update of do_cheby_iter<true> (evaluation_dg_laplacian.h
  * :485-502, :512-554):  r = A sol_old - rhs;  tmp = c_tm*tmp + c_np*diag*r;
- * sol_new = sol_old - tmp.  Supported for DG-SIPG and the continuous
- * Laplacian. */
+ * sol_new = sol_old - tmp.  Supported for DG-SIPG, the precomputed DG
+ * Laplacian and the continuous Laplacian. */
 sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, const void *diag,
                           void *sol_new, void *tmp, double c_np, double c_tm, void *stream);
 /* Fused low-storage RK stage of do_rk_stage<true> (evaluation_dg_advect.h

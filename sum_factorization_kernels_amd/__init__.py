@@ -18,11 +18,11 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libsfk.so")
 HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "sfk.h")
 
-CELL_LAPLACE, CONT_LAPLACE, DG_SIPG, DG_ADVECT = 0, 1, 2, 3
+CELL_LAPLACE, CONT_LAPLACE, DG_SIPG, DG_ADVECT, DG_PRECOMP = 0, 1, 2, 3, 4
 FP64, FP32 = 0, 1
 BASIS_HERMITE, BASIS_GLL = 0, 1
 GEOM_CARTESIAN, GEOM_PER_Q = 0, 1
-FLAG_REF_X_SWEEP, FLAG_VARIABLE_COEF, FLAG_VARIABLE_VELOCITY = 1, 2, 4
+FLAG_REF_X_SWEEP, FLAG_VARIABLE_COEF, FLAG_VARIABLE_VELOCITY, FLAG_REF_PRECOMP_COUPLING = 1, 2, 4, 8
 
 STATUS = {0: "SFK_OK", 1: "SFK_ERR_INVALID_ARG", 2: "SFK_ERR_UNSUPPORTED", 3: "SFK_ERR_HIP",
           4: "SFK_ERR_ALLOC", 5: "SFK_ERR_NONFINITE", 6: "SFK_ERR_NOT_READY", 7: "SFK_ERR_NO_CONVERGENCE"}
@@ -161,7 +161,7 @@ class Operator:
     def __init__(self, kind, degree, n_cells, precision=FP64, basis=None, geometry=GEOM_CARTESIAN,
                  flags=0, inv_jac=(1.0, 1.0, 1.0), velocity=(0.0, 0.0, 0.0), n_rhs=1):
         if basis is None:
-            basis = BASIS_HERMITE if kind in (CELL_LAPLACE, DG_SIPG) else BASIS_GLL
+            basis = BASIS_HERMITE if kind in (CELL_LAPLACE, DG_SIPG, DG_PRECOMP) else BASIS_GLL
         d = OpDesc()
         d.kind, d.degree, d.precision, d.basis, d.geometry, d.flags = kind, degree, precision, basis, geometry, flags
         d.n_cells[:] = list(n_cells)

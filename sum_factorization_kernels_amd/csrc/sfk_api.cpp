@@ -72,10 +72,26 @@ sfk_status upload(sfk_op *op, void **dev, const double *host, size_t count, size
 }
 
 // the operator's 1D tables in device memory, uploaded on the first launch
+// the precomputed DG Laplacian's kernels read their own table type
+static hipError_t upload_precomp(sfk_op *op, void **dev)
+{
+  Precomp1D pc;
+  if (!build_precomp_1d(op->d.degree, (op->d.flags & SFK_FLAG_REF_PRECOMP_COUPLING) != 0, pc))
+    return hipErrorInvalidValue;
+  // the reference's constants vanish for p >= 2 (not for p = 1, where
+  // phi_{n-2} = phi_0): only then can the kernel skip the neighbour layers
+  op->pc_couple = pc.v1 != 0. || pc.v2 != 0.;
+  const double *ij = op->d.inv_jac;
+  const double det = ij[0] * ij[1] * ij[2];
+  const double f[3] = {ij[0] * ij[0] / det, ij[1] * ij[1] / det, ij[2] * ij[2] / det};
+  return upload_precomp_tables(op->d.degree + 1, pc, f, op->d.precision == SFK_FP32, dev);
+}
+
 sfk_status ensure_device_tables(sfk_op *op)
 {
   if (op->dtab) return SFK_OK;
-  const hipError_t e = upload_device_tables(op->tab, op->d.precision == SFK_FP32, &op->dtab);
+  const hipError_t e = op->d.kind == SFK_DG_PRECOMP ? upload_precomp(op, &op->dtab)
+                                                     : upload_device_tables(op->tab, op->d.precision == SFK_FP32, &op->dtab);
   if (e != hipSuccess) {
     op->dtab = nullptr;
     set_error(std::string("device tables: ") + hipGetErrorString(e));
@@ -129,7 +145,7 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
   SFK_REQUIRE(desc && out, SFK_ERR_INVALID_ARG, "sfk_op_create: null argument");
   *out = nullptr;
   const sfk_op_desc &d = *desc;
-  SFK_REQUIRE(d.kind >= SFK_CELL_LAPLACE && d.kind <= SFK_DG_ADVECT, SFK_ERR_INVALID_ARG, "invalid operator kind");
+  SFK_REQUIRE(d.kind >= SFK_CELL_LAPLACE && d.kind <= SFK_DG_PRECOMP, SFK_ERR_INVALID_ARG, "invalid operator kind");
   SFK_REQUIRE(d.degree >= 1 && d.degree <= kMaxDegree, SFK_ERR_INVALID_ARG,
               "degree must be in 1.." + std::to_string(kMaxDegree));
   SFK_REQUIRE(d.precision == SFK_FP64 || d.precision == SFK_FP32, SFK_ERR_INVALID_ARG, "invalid precision");
@@ -164,6 +180,14 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
                 "variable coefficient: Laplacian operators only");
   if (d.flags & SFK_FLAG_VARIABLE_VELOCITY)
     SFK_REQUIRE(d.kind == SFK_DG_ADVECT, SFK_ERR_UNSUPPORTED, "variable velocity: advection only");
+  if (d.kind == SFK_DG_PRECOMP) {
+    SFK_REQUIRE(d.basis == SFK_BASIS_HERMITE, SFK_ERR_UNSUPPORTED,
+                "precomputed DG Laplacian uses the Hermite-like basis");
+    SFK_REQUIRE(d.geometry == SFK_GEOM_CARTESIAN, SFK_ERR_UNSUPPORTED, "precomputed DG Laplacian: Cartesian only");
+  }
+  if (d.flags & SFK_FLAG_REF_PRECOMP_COUPLING)
+    SFK_REQUIRE(d.kind == SFK_DG_PRECOMP, SFK_ERR_UNSUPPORTED,
+                "SFK_FLAG_REF_PRECOMP_COUPLING: precomputed DG Laplacian only");
 
   sfk_op *op = new sfk_op();
   op->d = d;
@@ -355,6 +379,15 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream)
     SFK_CHECK_HIP(launch_advect(L, s));
     return SFK_OK;
   }
+  case SFK_DG_PRECOMP: {
+    PrecompLaunch L;
+    SFK_ENSURE_TABLES(op);
+    fill_precomp_launch(op, L);
+    L.src = src;
+    L.dst = dst;
+    SFK_CHECK_HIP(launch_precomp(L, s));
+    return SFK_OK;
+  }
   }
   set_error("unknown operator kind");
   return SFK_ERR_INVALID_ARG;
@@ -429,6 +462,22 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
     }
     return SFK_OK;
   }
+  if (d.kind == SFK_DG_PRECOMP) {
+    SFK_REQUIRE(d.n_rhs == 1, SFK_ERR_UNSUPPORTED, "sfk_cheby_step: n_rhs must be 1");
+    PrecompLaunch L;
+    SFK_ENSURE_TABLES(op);
+    fill_precomp_launch(op, L);
+    L.cheby = true;
+    L.src = sol_old;
+    L.dst = sol_new;
+    L.rhs = rhs;
+    L.diag = diag;
+    L.tm = tmp;
+    L.c_np = c_np;
+    L.c_tm = c_tm;
+    SFK_CHECK_HIP(launch_precomp(L, s));
+    return SFK_OK;
+  }
   if (d.kind == SFK_CONT_LAPLACE || d.kind == SFK_CELL_LAPLACE) {
     SFK_REQUIRE(d.n_rhs == 1, SFK_ERR_UNSUPPORTED, "sfk_cheby_step: n_rhs must be 1");
     sfk_status st = ensure_scratch(op, (size_t)op->nvec * elem_size(op));
@@ -493,6 +542,14 @@ sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream)
     L.diagonal = true;
     L.dst = diag;
     SFK_CHECK_HIP(launch_sipg_general(L, s));
+    return SFK_OK;
+  }
+  if (d.kind == SFK_DG_PRECOMP) {
+    PrecompLaunch L;
+    SFK_ENSURE_TABLES(op);
+    fill_precomp_launch(op, L);
+    L.dst = diag;
+    SFK_CHECK_HIP(launch_precomp_diagonal(L, s));
     return SFK_OK;
   }
   set_error("sfk_compute_diagonal: not supported for advection");

@@ -19,6 +19,19 @@ struct Tables1D {
   double hermite_d = 0.;
 };
 
+// 1D matrices of the precomputed DG Laplacian on the unit interval
+// (legacy/include/precomputed_dg_laplacian.h:305-349): Hermite-like basis,
+// Gauss quadrature, penalty p(p+1).  mass, lap: dense [i*n+j]; v1, v2:
+// coupling of rows 0 and 1 to the left neighbour's dofs n-1 and n-2.
+struct Precomp1D {
+  std::vector<double> mass, lap;
+  double v1 = 0., v2 = 0.;
+};
+// ref_coupling: the reference's coupling expressions (:331-340), which
+// vanish; otherwise the SIPG face coupling.  False if the basis does not
+// have the coupling pattern the kernels assume.
+bool build_precomp_1d(int degree, bool ref_coupling, Precomp1D &pc);
+
 void gauss_rule(int n, std::vector<double> &pts, std::vector<double> &wts);
 std::vector<double> gll_points(int n);
 bool build_tables(int degree, int basis, Tables1D &t);

@@ -84,6 +84,28 @@ struct AdvectLaunch {
 };
 hipError_t launch_advect(const AdvectLaunch &L, hipStream_t stream);
 
+// DG Laplacian with precomputed 1D matrices (PrecomputedDGLaplacian)
+struct PrecompLaunch {
+  int n = 0;             // points per direction (degree + 1)
+  bool fp32 = false;
+  bool couple = true;    // neighbour coupling (false: both constants are zero)
+  bool cheby = false;
+  const void *src = nullptr;
+  void *dst = nullptr;
+  const void *rhs = nullptr, *diag = nullptr;
+  void *tm = nullptr;
+  double c_np = 0., c_tm = 0.;
+  long long ncells = 0;
+  int nc[3] = {1, 1, 1};
+  int nrhs = 1;
+  int cb = 0, order = 1;         // schedule
+  const void *ptab = nullptr;    // the operator's tables in device memory
+};
+hipError_t launch_precomp(const PrecompLaunch &L, hipStream_t stream);
+hipError_t launch_precomp_diagonal(const PrecompLaunch &L, hipStream_t stream);
+// the tables of launch_precomp in a new device allocation (f: direction factors)
+hipError_t upload_precomp_tables(int n, const Precomp1D &pc, const double f[3], bool fp32, void **dev);
+
 // continuous Laplacian: coefficient gradient g_q = JxW grad u . grad lambda
 struct CoefGradLaunch {
   const Tables1D *tables = nullptr;

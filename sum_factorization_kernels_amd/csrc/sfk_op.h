@@ -22,6 +22,9 @@ struct sfk_op {
   // workgroup schedule (sfk_schedule): cells per block (0 = default), block
   // order, continuous scatter (-1 automatic, 0 atomic, 1 colored)
   int sched_cb = 0, sched_order = 1, cont_scatter = -1;
+  // precomputed DG Laplacian: neighbour coupling constants not both zero
+  // (set with the device tables)
+  bool pc_couple = true;
 };
 
 namespace sfk {
@@ -54,6 +57,19 @@ inline void fill_advect_launch(const sfk_op *op, AdvectLaunch &L)
     L.face_vel = op->face_vel;
   }
   L.dtab = op->dtab;
+  L.cb = op->sched_cb;
+  L.order = op->sched_order;
+}
+
+inline void fill_precomp_launch(const sfk_op *op, PrecompLaunch &L)
+{
+  L.n = op->d.degree + 1;
+  L.fp32 = op->d.precision == SFK_FP32;
+  L.couple = op->pc_couple;
+  L.ncells = op->ncells;
+  for (int d = 0; d < 3; ++d) L.nc[d] = op->d.n_cells[d];
+  L.nrhs = op->d.n_rhs;
+  L.ptab = op->dtab;
   L.cb = op->sched_cb;
   L.order = op->sched_order;
 }

@@ -231,6 +231,59 @@ std::vector<double> gll_points(int n)
   return p;
 }
 
+bool build_precomp_1d(int degree, bool ref_coupling, Precomp1D &pc)
+{
+  if (degree < 1 || degree > kMaxDegree) return false;
+  const int n = degree + 1;
+  std::vector<double> x, w;
+  gauss_rule(n, x, w);
+  HermiteLike hb(degree);
+  std::vector<double> val(n * n), der(n * n), v0(n), v1(n), d0(n), d1(n);
+  for (int i = 0; i < n; ++i) {
+    for (int q = 0; q < n; ++q) {
+      val[i * n + q] = hb.value(i, x[q]);
+      der[i * n + q] = hb.derivative(i, x[q]);
+    }
+    v0[i] = hb.value(i, 0.);
+    v1[i] = hb.value(i, 1.);
+    d0[i] = hb.derivative(i, 0.);
+    d1[i] = hb.derivative(i, 1.);
+  }
+  const double sigma = double(degree) * n;
+  pc.mass.assign(n * n, 0.);
+  pc.lap.assign(n * n, 0.);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double sm = 0., sl = 0.;
+      for (int q = 0; q < n; ++q) {
+        sm += val[i * n + q] * val[j * n + q] * w[q];
+        sl += der[i * n + q] * der[j * n + q] * w[q];
+      }
+      // the cell's own traces on its faces x = 0 (outer normal -1) and x = 1
+      sl += sigma * v0[i] * v0[j] + 0.5 * d0[i] * v0[j] + 0.5 * d0[j] * v0[i];
+      sl += sigma * v1[i] * v1[j] - 0.5 * d1[i] * v1[j] - 0.5 * d1[j] * v1[i];
+      pc.mass[i * n + j] = sm;
+      pc.lap[i * n + j] = sl;
+    }
+  // C(i, j): test phi_i of a cell at its x = 0 against trial phi_j of its
+  // left neighbour at x = 1: 1/2 phi_j'(1) phi_i(0) - 1/2 phi_j(1) phi_i'(0)
+  // - sigma phi_i(0) phi_j(1).  Only rows 0, 1 and columns n-2, n-1 can be
+  // non-zero, with C(0, n-2) = C(1, n-1) and C(1, n-2) = 0 (mirror
+  // symmetry); the kernels rely on that pattern.
+  auto C = [&](int i, int j) { return 0.5 * d1[j] * v0[i] - 0.5 * v1[j] * d0[i] - sigma * v0[i] * v1[j]; };
+  const int a = n - 1, b = n - 2;
+  const double s1 = std::fabs(C(0, a));
+  if (std::fabs(C(0, b) - C(1, a)) > 1e-12 * s1 || (n > 2 && std::fabs(C(1, b)) > 1e-12 * s1)) return false;
+  if (ref_coupling) {
+    pc.v1 = -v1[0] * v0[a] * sigma + 0.5 * d1[0] * v0[a] - 0.5 * d0[a] * v1[0];
+    pc.v2 = -v1[0] * v0[b] * sigma + 0.5 * d1[0] * v0[b] - 0.5 * d0[b] * v1[0];
+  } else {
+    pc.v1 = C(0, a);
+    pc.v2 = C(0, b);
+  }
+  return true;
+}
+
 bool build_tables(int degree, int basis, Tables1D &t)
 {
   if (degree < 1 || degree > kMaxDegree) return false;

@@ -5,6 +5,7 @@
 //   EvaluationCellLaplacian<dim,degree,Number>  legacy/include/evaluation_cell_laplacian.h:26-908
 //   EvaluationDGLaplacian<dim,degree,Number>    legacy/include/evaluation_dg_laplacian.h:33-716
 //   EvaluationDGAdvection<dim,degree,Number>    legacy/include/evaluation_dg_advect.h:96-845
+//   PrecomputedDGLaplacian<dim,degree,Number>   legacy/include/precomputed_dg_laplacian.h:33-389
 //
 // As in the reference, an evaluator owns its vectors (here in device memory,
 // canonical cell-major layout) and initialize() sets the initial data.  Only
@@ -235,6 +236,73 @@ public:
 
   unsigned int n_cells[3] = {1, 1, 1};
   sfk::DeviceVector<Number> sol_old, sol_new, sol_rhs, sol_tmp, mat_diagonal;
+
+private:
+  std::unique_ptr<sfk::Operator> op_;
+};
+
+// --------------------------------------------------------------------------
+// PrecomputedDGLaplacian (legacy/include/precomputed_dg_laplacian.h:33-389):
+// DG Laplacian of the Hermite-like basis from precomputed 1D matrices on
+// unit cells, periodic.  ref_coupling = true before initialize() reproduces
+// the reference's neighbour coupling constants (which vanish); the default
+// couples neighbouring cells through the SIPG face terms.
+template <int dim, int degree, typename Number>
+class PrecomputedDGLaplacian {
+  static_assert(dim == 3, "three-dimensional operators only");
+
+public:
+  static constexpr unsigned int dimension = dim;
+  static constexpr unsigned int n_q_points = (degree + 1) * (degree + 1) * (degree + 1);
+  static constexpr unsigned int dofs_per_cell = n_q_points;
+  unsigned int blx = 1, bly = 1, blz = 1;
+  bool ref_coupling = false;
+
+  // n_cells_in[0] counts cells (the reference divides it by its SIMD width)
+  void initialize(const unsigned int *n_cells_in)
+  {
+    for (int d = 0; d < 3; ++d) n_cells[d] = n_cells_in[d];
+    const double unit[3] = {1., 1., 1.};
+    sfk_op_desc d = sfk::make_desc(SFK_DG_PRECOMP, degree, sfk::precision_of<Number>(), SFK_BASIS_HERMITE, n_cells,
+                                   unit);
+    if (ref_coupling) d.flags |= SFK_FLAG_REF_PRECOMP_COUPLING;
+    op_.reset(new sfk::Operator(d));
+    const std::size_t n = n_elements() * dofs_per_cell;
+    input_array.resize(n);
+    output_array.resize(n);
+    // initial data of precomputed_dg_laplacian.h:57-80
+    input_array.fill(1.);
+    output_array.fill(0.);
+  }
+  std::size_t n_elements() const { return std::size_t(n_cells[0]) * n_cells[1] * n_cells[2]; }
+
+  // The reference has no check (:90-93).  Here: max |A 1| / max |A x| for a
+  // fixed non-constant x, which vanishes to rounding for the SIPG coupling
+  // (the basis is a partition of unity); input_array and output_array are
+  // overwritten.
+  double do_verification()
+  {
+    std::vector<Number> x(input_array.size());
+    for (std::size_t i = 0; i < x.size(); ++i) x[i] = Number(std::sin(0.37 * double(i % 1013)));
+    input_array.upload(x);
+    matrix_vector_product();
+    double scale = 0.;
+    for (Number v : output_array.download()) scale = std::max(scale, std::abs(double(v)));
+    input_array.fill(1.);
+    matrix_vector_product();
+    double r = 0.;
+    for (Number v : output_array.download()) r = std::max(r, std::abs(double(v)));
+    return scale > 0. ? r / scale : r;
+  }
+
+  void matrix_vector_product()
+  {
+    sfk::check(sfk_apply(op_->get(), input_array.data(), output_array.data(), nullptr));
+    sfk::check(sfk_synchronize(nullptr));
+  }
+
+  unsigned int n_cells[3] = {1, 1, 1};
+  sfk::DeviceVector<Number> input_array, output_array;
 
 private:
   std::unique_ptr<sfk::Operator> op_;

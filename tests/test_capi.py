@@ -61,6 +61,9 @@ def test_descriptor_validation(bad):
     (dict(kind=sfk.DG_SIPG, n_rhs=2), 2),
     (dict(kind=sfk.DG_SIPG, geometry=sfk.GEOM_PER_Q, flags=sfk.FLAG_REF_X_SWEEP), 2),
     (dict(kind=sfk.DG_ADVECT, flags=sfk.FLAG_VARIABLE_COEF), 2),
+    (dict(kind=sfk.DG_PRECOMP, basis=sfk.BASIS_GLL), 2),
+    (dict(kind=sfk.DG_PRECOMP, geometry=sfk.GEOM_PER_Q), 2),
+    (dict(kind=sfk.DG_SIPG, flags=sfk.FLAG_REF_PRECOMP_COUPLING), 2),
 ])
 def test_unsupported_combinations(kw, status):
     base = dict(degree=3, n_cells=(2, 2, 2), inv_jac=(2.0, 2.0, 2.0))
@@ -75,6 +78,8 @@ def test_vector_sizes():
     assert op.vector_size == 65 ** 3  # 274,625 nodes of the Q4 16^3 grid
     op = sfk.Operator(sfk.DG_SIPG, 5, (32, 32, 32), geometry=sfk.GEOM_PER_Q)
     assert op.vector_size == 7077888 and op.n_cell_q == 7077888 and op.n_face_q == 32 ** 3 * 36
+    op = sfk.Operator(sfk.DG_PRECOMP, 3, (56, 50, 55))
+    assert op.vector_size == 56 * 50 * 55 * 64  # the reference driver's default grid at degree 3
 
 
 def test_setters_validate_sizes():
@@ -89,7 +94,8 @@ def test_setters_validate_sizes():
 @pytest.mark.parametrize("degree,kind,geom,default_cb", [
     (5, sfk.DG_SIPG, sfk.GEOM_PER_Q, 3), (5, sfk.DG_SIPG, sfk.GEOM_CARTESIAN, 7),
     (4, sfk.CONT_LAPLACE, sfk.GEOM_CARTESIAN, 10), (8, sfk.CELL_LAPLACE, sfk.GEOM_CARTESIAN, 3),
-    (3, sfk.DG_ADVECT, sfk.GEOM_CARTESIAN, 16)])
+    (3, sfk.DG_ADVECT, sfk.GEOM_CARTESIAN, 16), (3, sfk.DG_PRECOMP, sfk.GEOM_CARTESIAN, 16),
+    (8, sfk.DG_PRECOMP, sfk.GEOM_CARTESIAN, 3)])
 def test_schedule_defaults_and_validation(degree, kind, geom, default_cb):
     kw = {"basis": sfk.BASIS_GLL} if kind in (sfk.CONT_LAPLACE, sfk.DG_ADVECT) else {}
     if kind == sfk.DG_ADVECT:
