@@ -5,7 +5,7 @@
 Headline workload (one JSON line on stdout): DG-SIPG Laplacian, degree 5
 (6^3 dofs per cell), 32^3 cells of [0,1]^3 deformed by
 x + 0.1 sin(pi x) sin(pi y) sin(pi z) (1,1,1) with the per-quadrature-point
-metric stored (cell: 6 values per q-point, faces: 8 per face q-point), FP64,
+metric stored (cell: 6 values per q-point, faces: 7 per face q-point), FP64,
 input standard normal from seed 1; a "step" is one matvec, value = GDoF/s.
 The same line carries the FP32 matvec, one degree-5 point-Jacobi Chebyshev
 sweep, and (unless --no-extras) the other workloads: continuous Q4 16^3
@@ -55,9 +55,9 @@ def timed(fn, steps, warmup):
 def sipg_bytes_per_dof(degree, elem):
     """algorithmic bytes per dof of the per-q-metric SIPG matvec: src + dst,
     the cell metric (6 per q-point) and the face metric (3 faces per cell,
-    8 values per face q-point), all at the operator precision"""
+    7 values per face q-point), all at the operator precision"""
     n = degree + 1
-    return elem * (2 + 6 + 3 * 8 * n * n / n ** 3)
+    return elem * (2 + 6 + 3 * 7 * n * n / n ** 3)
 
 
 def power_lmax(op, dinv, n, iters=15):

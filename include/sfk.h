@@ -104,10 +104,10 @@ sfk_status sfk_op_qdata_sizes(const sfk_op *op, long long *n_cell_q, long long *
  *    symmetric G = JxW * J^{-1} J^{-T} (JxW including the quadrature weight);
  *  face metric, SFK_GEOM_PER_Q (DG-SIPG): for every face direction d and
  *    every cell c, the face between c and its upper neighbour in d:
- *    8 values per face q-point, layout [d][cell][component][q], components
- *    (aL0,aL1,aL2, aU0,aU1,aU2, jxw, sigma) with aX = JxW_f J_X^{-1} n,
- *    n the unit normal pointing from c to its neighbour, jxw = JxW_f and
- *    sigma = penalty * JxW_f;
+ *    7 values per face q-point, layout [d][cell][component][q], components
+ *    (aL0,aL1,aL2, aU0,aU1,aU2, sigma) with aX = JxW_f J_X^{-1} n,
+ *    n the unit normal pointing from c to its neighbour, JxW_f the face
+ *    Jacobian times the quadrature weight and sigma = penalty * JxW_f;
  *  coefficient (SFK_FLAG_VARIABLE_COEF): k per cell q-point [cell][q];
  *  velocity (SFK_FLAG_VARIABLE_VELOCITY): cell part [cell][3][q]
  *    (a_x,a_y,a_z at cell q-points), face part [d][cell][q] = a.n at the

@@ -192,8 +192,8 @@ void sfko_cont_laplacian_coef_grad(int degree, const int nc[3], const double inv
 /*   a(u,v) = sum_K (G grad^ u, grad^ v)_K                                */
 /*          + sum_F [ -{du/dn}[v] - {dv/dn}[u] + sigma [u][v] ]_F          */
 /* cell metric [cell][6][q] (G00,G11,G22,G01,G02,G12) with G = JxW        */
-/* J^{-1}J^{-T}; face metric [d][cell][8][fa] for the face between cell   */
-/* and its upper neighbour in d: aL(3), aU(3), jxw, sigma, where          */
+/* J^{-1}J^{-T}; face metric [d][cell][7][fa] for the face between cell   */
+/* and its upper neighbour in d: aL(3), aU(3), sigma, where               */
 /* aX = JxW_f J_X^{-1} n (n from lower to upper cell) and JxW_f includes   */
 /* the face quadrature weight.  The face value of the neighbour and its   */
 /* normal derivative use the Hermite trick of evaluation_dg_laplacian.h   */
@@ -244,7 +244,7 @@ static void sipg_general_cell(const sfko_tables *t, const int nc[3], const doubl
       apply_dir(n, 2, 1, t->D, 0, up, gp[hi]);
       /* face data: own side is L for s = 1 (face of this cell), U for s = 0 */
       const long owner = s ? c : nb;
-      const double *f = fm + ((long)d * ncell + owner) * 8 * n2;
+      const double *f = fm + ((long)d * ncell + owner) * 7 * n2;
       const double sn = s ? 1. : -1.;
       double tv[MAXN * MAXN], td[3][MAXN * MAXN];
       for (int fa = 0; fa < n2; ++fa) {
@@ -254,7 +254,7 @@ static void sipg_general_cell(const sfko_tables *t, const int nc[3], const doubl
           ao[e] = s ? aL[e * n2 + fa] : aU[e * n2 + fa];
           an[e] = s ? aU[e * n2 + fa] : aL[e * n2 + fa];
         }
-        const double sigma = f[7 * n2 + fa];
+        const double sigma = f[6 * n2 + fa];
         double dn = 0;
         for (int e = 0; e < 3; ++e) dn += ao[e] * gm[e][fa] + an[e] * gp[e][fa];
         const double avg = sn * 0.5 * dn; /* {du/dn_out} JxW_f */
@@ -303,14 +303,13 @@ void sfko_cartesian_metric(int degree, const int nc[3], const double inv_jac[3],
   for (int d = 0; d < 3; ++d)
     for (long c = 0; c < ncell; ++c)
       for (int fa = 0; fa < n2; ++fa) {
-        double *f = face_metric + ((long)d * ncell + c) * 8 * n2;
+        double *f = face_metric + ((long)d * ncell + c) * 7 * n2;
         const double jf = jxw * inv_jac[d] * t.w[fa % n] * t.w[fa / n];
         for (int e = 0; e < 3; ++e) {
           f[e * n2 + fa] = e == d ? jf * inv_jac[d] : 0.;
           f[(3 + e) * n2 + fa] = e == d ? jf * inv_jac[d] : 0.;
         }
-        f[6 * n2 + fa] = jf;
-        f[7 * n2 + fa] = (degree + 1) * (degree + 1) * inv_jac[d] * jf;
+        f[6 * n2 + fa] = (degree + 1) * (degree + 1) * inv_jac[d] * jf;
       }
 }
 

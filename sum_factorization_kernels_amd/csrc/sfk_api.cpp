@@ -253,7 +253,7 @@ sfk_status sfk_op_set_face_metric(sfk_op *op, const double *host, size_t count)
   SFK_REQUIRE(op, SFK_ERR_INVALID_ARG, "null handle");
   SFK_REQUIRE(op->d.geometry == SFK_GEOM_PER_Q && op->d.kind == SFK_DG_SIPG, SFK_ERR_INVALID_ARG,
               "face metric needs DG-SIPG with SFK_GEOM_PER_Q");
-  return upload(op, &op->face_metric, host, count, (size_t)op->nfq * 3 * 8, "face metric");
+  return upload(op, &op->face_metric, host, count, (size_t)op->nfq * 3 * 7, "face metric");
 }
 
 sfk_status sfk_op_set_coefficient(sfk_op *op, const double *host, size_t count)
@@ -320,7 +320,7 @@ sfk_status sfk_op_set_sine_deformation(sfk_op *op, double amp)
   const size_t es = elem_size(op);
   if (!op->cell_metric) SFK_CHECK_HIP(hipMalloc(&op->cell_metric, (size_t)op->ncq * 6 * es));
   if (op->d.kind == SFK_DG_SIPG && !op->face_metric)
-    SFK_CHECK_HIP(hipMalloc(&op->face_metric, (size_t)op->nfq * 3 * 8 * es));
+    SFK_CHECK_HIP(hipMalloc(&op->face_metric, (size_t)op->nfq * 3 * 7 * es));
   SFK_CHECK_HIP(launch_sine_metric(op->tab, op->d.precision == SFK_FP32, op->d.n_cells, amp, op->cell_metric,
                                    op->d.kind == SFK_DG_SIPG ? op->face_metric : nullptr, nullptr));
   SFK_CHECK_HIP(hipDeviceSynchronize());

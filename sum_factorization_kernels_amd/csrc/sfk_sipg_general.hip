@@ -112,14 +112,14 @@ __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, c
       }
     }
     const long long owner = s ? cell : neighbor_cell(cell, a.nx, a.ny, a.nz, DIR, 0);
-    const T *fmp = a.fm + ((long long)DIR * a.ncells + owner) * 8 * N2 + l;
+    const T *fmp = a.fm + ((long long)DIR * a.ncells + owner) * 7 * N2 + l;
     T ao[3], an[3];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
       ao[e] = s ? fmp[e * N2] : fmp[(3 + e) * N2];
       an[e] = s ? fmp[(3 + e) * N2] : fmp[e * N2];
     }
-    const T sigma = fmp[7 * N2];
+    const T sigma = fmp[6 * N2];
     const T sn = s ? T(1) : T(-1);
     T dn = 0;
 #pragma unroll
@@ -180,28 +180,33 @@ sipg_general_kernel(const GenArgs<N, T> a)
     lds_write_line<N, N, 2>(Q, l, v);
   }
   if constexpr (!DIAG) {
-    for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
-      const int cc = it / (6 * N2), f = (it / N2) % 6, fa = it % N2;
-      const long long cl = item0 + cc;
-      if (cl >= a.ncells) continue;
-      const int d = f / 2, s = f % 2;
-      const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, s) * N3;
-      const int lf = s ? 0 : p, li = s ? 1 : p - 1;
-      int of, oi;
-      if (d == 0) {
-        of = face_dof<N, 0>(lf, fa);
-        oi = face_dof<N, 0>(li, fa);
-      } else if (d == 1) {
-        of = face_dof<N, 1>(lf, fa);
-        oi = face_dof<N, 1>(li, fa);
-      } else {
-        of = face_dof<N, 2>(lf, fa);
-        oi = face_dof<N, 2>(li, fa);
+    // neighbour layers: thread l of a cell loads face point l of all six
+    // faces (twelve independent loads, issued together)
+    if (act) {
+      const int ci = (int)cell;
+      const int ix = ci % a.nx, r = ci / a.nx, iy = r % a.ny, iz = r / a.ny;
+      const int sx = 1, sy = a.nx, sz = a.nx * a.ny;
+      // periodic neighbour offsets (in cells) below / above in x, y, z
+      const int off[6] = {ix == 0 ? (a.nx - 1) * sx : -sx, ix == a.nx - 1 ? -(a.nx - 1) * sx : sx,
+                          iy == 0 ? (a.ny - 1) * sy : -sy, iy == a.ny - 1 ? -(a.ny - 1) * sy : sy,
+                          iz == 0 ? (a.nz - 1) * sz : -sz, iz == a.nz - 1 ? -(a.nz - 1) * sz : sz};
+      T uf[6], ui[6];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int d = f / 2, s = f % 2;
+        const T *nb = a.src + (long long)(ci + off[f]) * N3;
+        const int lf = s ? 0 : p, li = s ? 1 : p - 1;
+        const int of = d == 0 ? face_dof<N, 0>(lf, l) : (d == 1 ? face_dof<N, 1>(lf, l) : face_dof<N, 2>(lf, l));
+        const int oi = d == 0 ? face_dof<N, 0>(li, l) : (d == 1 ? face_dof<N, 1>(li, l) : face_dof<N, 2>(li, l));
+        uf[f] = nb[of];
+        ui[f] = nb[oi];
       }
-      const T uf = a.src[nb + of], ui = a.src[nb + oi];
-      sF[(cc * 12 + f) * N2 + fa] = uf;
       const T hd = ctab(a.dt).hd;
-      sF[(cc * 12 + 6 + f) * N2 + fa] = (s ? -hd : hd) * (ui - uf);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        F[f * N2 + l] = uf[f];
+        F[(6 + f) * N2 + l] = (f % 2 ? -hd : hd) * (ui[f] - uf[f]);
+      }
     }
   }
   __syncthreads();
@@ -570,13 +575,12 @@ __global__ void metric_face_kernel(const MetricArgs a, int write)
     area += jxwf;
     if (!write) continue;
     for (int e = 0; e < 3; ++e) nv[e] /= nrm;
-    const long long b = ((long long)d * a.ncells + c) * 8 * n2 + fa;
+    const long long b = ((long long)d * a.ncells + c) * 7 * n2 + fa;
     for (int e = 0; e < 3; ++e) {
       store(a.fm, b + e * n2, jxwf * (JLi[e][0] * nv[0] + JLi[e][1] * nv[1] + JLi[e][2] * nv[2]), a.fp32);
       store(a.fm, b + (3 + e) * n2, jxwf * (JUi[e][0] * nv[0] + JUi[e][1] * nv[1] + JUi[e][2] * nv[2]), a.fp32);
     }
-    store(a.fm, b + 6 * n2, jxwf, a.fp32);
-    store(a.fm, b + 7 * n2, tau * jxwf, a.fp32);
+    store(a.fm, b + 6 * n2, tau * jxwf, a.fp32);
   }
   if (!write) a.area[(long long)d * a.ncells + c] = area;
 }

@@ -74,7 +74,7 @@ def sipg_general_dense(degree, nc, cell_metric, face_metric):
         K = np.einsum("aiq,qab,bjq->ij", gi, G, gi)
         A[c * n3:(c + 1) * n3, c * n3:(c + 1) * n3] += K
     # face terms
-    fm = face_metric.reshape(3, ncell, 8, n2)
+    fm = face_metric.reshape(3, ncell, 7, n2)
     for d in range(3):
         lo, hi = (1, 2) if d == 0 else ((0, 2) if d == 1 else (0, 1))
         for c in range(ncell):
@@ -83,7 +83,7 @@ def sipg_general_dense(degree, nc, cell_metric, face_metric):
             xu[d] = (xu[d] + 1) % nc[d]
             u = cell_id(nc, *xu)
             aL, aU = fm[d, c, 0:3], fm[d, c, 3:6]
-            sigma = fm[d, c, 7]
+            sigma = fm[d, c, 6]
             for fa in range(n2):
                 a_lo, a_hi = fa % n, fa // n
                 # traces and reference gradients of the L (at xi_d=1) and U (xi_d=0) bases

@@ -71,7 +71,7 @@ def sine_metric(degree, nc, amp):
     cm = np.stack([G[..., 0, 0], G[..., 1, 1], G[..., 2, 2], G[..., 0, 1], G[..., 0, 2], G[..., 1, 2]], axis=1)
     cm = np.ascontiguousarray(cm.reshape(ncell, 6, n ** 3)).ravel()
     # faces
-    fm = np.zeros((3, ncell, 8, n * n))
+    fm = np.zeros((3, ncell, 7, n * n))
     vol = np.sum((det * wq), axis=1)  # cell volumes
     for d in range(3):
         lo, hi = (1, 2) if d == 0 else ((0, 2) if d == 1 else (0, 1))
@@ -105,8 +105,7 @@ def sine_metric(degree, nc, amp):
         tau = (degree + 1) ** 2 * 0.5 * (farea / vol + farea / volU)
         fm[d, :, 0:3, :] = np.transpose(aL, (0, 2, 1))
         fm[d, :, 3:6, :] = np.transpose(aU, (0, 2, 1))
-        fm[d, :, 6, :] = jxwf
-        fm[d, :, 7, :] = tau[:, None] * jxwf
+        fm[d, :, 6, :] = tau[:, None] * jxwf
     return cm, np.ascontiguousarray(fm).ravel()
 
 

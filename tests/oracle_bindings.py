@@ -261,7 +261,7 @@ def cartesian_metric(degree, nc, inv_jac):
     n = degree + 1
     ncell = int(np.prod(nc))
     cm = np.zeros(ncell * 6 * n ** 3)
-    fm = np.zeros(3 * ncell * 8 * n * n)
+    fm = np.zeros(3 * ncell * 7 * n * n)
     oracle().sfko_cartesian_metric(degree, _i3(nc), _d3(inv_jac), _ptr(cm), _ptr(fm))
     return cm, fm
 
