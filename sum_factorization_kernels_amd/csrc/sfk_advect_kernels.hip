@@ -49,16 +49,16 @@ struct AdvArgs {
 };
 
 // normal velocity a.n (outward from the cell) at face point fa of face (d, s)
+// (the lower face's value is stored with the lower neighbour `lower`)
 template <int N, typename T, bool VAR>
-__device__ __forceinline__ T normal_velocity(const AdvArgs<N, T> &a, long long cell, int d, int s, int fa)
+__device__ __forceinline__ T normal_velocity(const AdvArgs<N, T> &a, long long cell, int lower, int d, int s, int fa)
 {
   constexpr int N2 = N * N;
   if constexpr (!VAR) {
     return s ? a.vel[d] : -a.vel[d];
   } else {
     if (s) return a.face_vel[((long long)d * a.ncells + cell) * N2 + fa];
-    const long long nb = neighbor_cell(cell, a.nx, a.ny, a.nz, d, 0);
-    return -a.face_vel[((long long)d * a.ncells + nb) * N2 + fa];
+    return -a.face_vel[((long long)d * a.ncells + lower) * N2 + fa];
   }
 }
 
@@ -73,26 +73,22 @@ __device__ __forceinline__ T cell_velocity(const AdvArgs<N, T> &a, long long cel
   }
 }
 
-// load the value face layers of the 6 neighbours (GLL nodal: the face layer
-// is the trace) into F[cc][f][fa]
+// value face layers of the 6 neighbours (GLL nodal: the face layer is the
+// trace) into this cell's F[f][fa]: thread l loads face point l of every
+// face (six independent loads, issued together)
 template <int N, typename T>
-__device__ __forceinline__ void load_face_layers(const AdvArgs<N, T> &a, const T *src, T *sF, long long cell0,
-                                                 int CB, int nf_first, int nf_count)
+__device__ __forceinline__ void load_face_layers(const T *src, T *F, const int (&nbr)[6], int l)
 {
   constexpr int N2 = N * N, N3 = N * N * N, p = N - 1;
-  for (int it = threadIdx.x; it < CB * nf_count * N2; it += blockDim.x) {
-    const int cc = it / (nf_count * N2), fi = (it / N2) % nf_count, fa = it % N2;
-    const long long cl = cell0 + cc;
-    if (cl >= a.ncells) continue;
-    const int f = nf_first + fi, d = f / 2, s = f % 2;
-    const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, s) * N3;
-    const int lf = s ? 0 : p;
-    int of;
-    if (d == 0) of = face_dof<N, 0>(lf, fa);
-    else if (d == 1) of = face_dof<N, 1>(lf, fa);
-    else of = face_dof<N, 2>(lf, fa);
-    sF[(cc * 6 + fi) * N2 + fa] = src[nb + of];
+  T v[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int d = f / 2, lf = f % 2 ? 0 : p;
+    const int of = d == 0 ? face_dof<N, 0>(lf, l) : (d == 1 ? face_dof<N, 1>(lf, l) : face_dof<N, 2>(lf, l));
+    v[f] = src[(long long)nbr[f] * N3 + of];
   }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) F[f * N2 + l] = v[f];
 }
 
 template <int N, typename T, typename E>
@@ -114,7 +110,7 @@ __device__ __forceinline__ void face_sweep_t(const E &M, T *F, int narr, int l, 
 // forward (kind 0) and transposed weak form (kind 1): direction phase
 template <int N, int DIR, typename T, bool VAR, bool ADJ>
 __device__ __forceinline__ void adv_dir(const AdvArgs<N, T> &a, const T *Acell, const T *F, int l, long long cell,
-                                        T (&r)[N])
+                                        int lower, T (&r)[N])
 {
   const auto &tb = ctab(a.dt);
   T q[N];
@@ -146,7 +142,7 @@ __device__ __forceinline__ void adv_dir(const AdvArgs<N, T> &a, const T *Acell, 
   for (int s = 0; s < 2; ++s) {
     const T um = s ? v1 : v0;
     const T up = F[(2 * DIR + s) * N2 + l];
-    const T an = normal_velocity<N, T, VAR>(a, cell, DIR, s, l);
+    const T an = normal_velocity<N, T, VAR>(a, cell, lower, DIR, s, l);
     if constexpr (!ADJ) {
       // -0.5((u^- + u^+) a.n + |a.n| (u^- - u^+)), evaluation_dg_advect.h:403-404
       fl[s] = T(-0.5) * ((um + up) * an + fabs(an) * (um - up)) * wf;
@@ -175,6 +171,8 @@ advect_kernel(const AdvArgs<N, T> a)
   T *A = sA + (act ? c : 0) * N3;
   T *B = sB + (act ? c : 0) * N3;
   T *F = sF + (KIND != 2 && act ? c * 6 * N2 : 0);
+  int nbr[6] = {0, 0, 0, 0, 0, 0}; // periodic neighbour cells
+  if (KIND != 2 && act) periodic_neighbors((int)cell, a.nx, a.ny, a.nz, nbr);
 
   if constexpr (KIND == 2) {
     // M^{-1} x = Sinv^T W^{-1} Sinv x
@@ -238,18 +236,17 @@ advect_kernel(const AdvArgs<N, T> a)
       eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 2>(B, l, v);
     }
-    for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
-      const int cc = it / (6 * N2), fi = (it / N2) % 6, fa = it % N2;
-      const long long cl = cell0 + cc;
-      if (cl >= a.ncells) continue;
-      const int d = fi % 3;
-      const T *vec = fi < 3 ? a.src : a.aux;
-      const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, 1) * N3;
-      int of;
-      if (d == 0) of = face_dof<N, 0>(0, fa);
-      else if (d == 1) of = face_dof<N, 1>(0, fa);
-      else of = face_dof<N, 2>(0, fa);
-      sF[(cc * 6 + fi) * N2 + fa] = vec[nb + of];
+    if (act) {
+      T v[6];
+#pragma unroll
+      for (int fi = 0; fi < 6; ++fi) {
+        const int d = fi % 3;
+        const T *vec = fi < 3 ? a.src : a.aux;
+        const int of = d == 0 ? face_dof<N, 0>(0, l) : (d == 1 ? face_dof<N, 1>(0, l) : face_dof<N, 2>(0, l));
+        v[fi] = vec[(long long)nbr[2 * d + 1] * N3 + of];
+      }
+#pragma unroll
+      for (int fi = 0; fi < 6; ++fi) F[fi * N2 + l] = v[fi];
     }
     __syncthreads();
     if (act) {
@@ -318,7 +315,7 @@ advect_kernel(const AdvArgs<N, T> a)
       eo_apply<N, 1>(tb.Sf, u, v);
       lds_write_line<N, N, 2>(A, l, v);
     }
-    load_face_layers<N, T>(a, a.src, sF, cell0, CB, 0, 6);
+    if (act) load_face_layers<N, T>(a.src, F, nbr, l);
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
@@ -341,14 +338,14 @@ advect_kernel(const AdvArgs<N, T> a)
     if (act) {
       const auto &tb = ctab(a.dt);
       T r[N];
-      adv_dir<N, 0, T, VAR, ADJ>(a, A, F, l, cell, r);
+      adv_dir<N, 0, T, VAR, ADJ>(a, A, F, l, cell, nbr[0], r);
       lds_write_line<N, N, 0>(B, l, r);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T r[N], b[N];
-      adv_dir<N, 1, T, VAR, ADJ>(a, A, F, l, cell, r);
+      adv_dir<N, 1, T, VAR, ADJ>(a, A, F, l, cell, nbr[2], r);
       lds_read_line<N, N, 1>(B, l, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) r[i] += b[i];
@@ -358,7 +355,7 @@ advect_kernel(const AdvArgs<N, T> a)
     if (act) {
       const auto &tb = ctab(a.dt);
       T r[N], b[N], o[N];
-      adv_dir<N, 2, T, VAR, ADJ>(a, A, F, l, cell, r);
+      adv_dir<N, 2, T, VAR, ADJ>(a, A, F, l, cell, nbr[4], r);
       lds_read_line<N, N, 2>(B, l, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) r[i] += b[i];

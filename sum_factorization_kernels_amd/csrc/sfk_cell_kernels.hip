@@ -243,29 +243,29 @@ cell_kernel(const CellArgs<N, T> a)
     }
     if constexpr (FACES) {
       // neighbour layers: value layer and Hermite normal derivative
-      // w0*(inner - face) (evaluation_dg_laplacian.h:196-203, 272-283)
-      for (int it = t; it < CB * 6 * N2; it += blockDim.x) {
-        const int cc = it / (6 * N2), f = (it / N2) % 6, fa = it % N2;
-        const long long cl = cell0 + cc;
-        if (cl >= a.ncells) continue;
-        const int d = f / 2, s = f % 2;
-        const long long nb = neighbor_cell(cl, a.nx, a.ny, a.nz, d, s) * N3;
-        const int lf = s ? 0 : p, li = s ? 1 : p - 1;
-        int of, oi;
-        if (d == 0) {
-          of = face_dof<N, 0>(lf, fa);
-          oi = face_dof<N, 0>(li, fa);
-        } else if (d == 1) {
-          of = face_dof<N, 1>(lf, fa);
-          oi = face_dof<N, 1>(li, fa);
-        } else {
-          of = face_dof<N, 2>(lf, fa);
-          oi = face_dof<N, 2>(li, fa);
+      // w0*(inner - face) (evaluation_dg_laplacian.h:196-203, 272-283);
+      // thread l of a cell loads face point l of all six faces (twelve
+      // independent loads, issued together)
+      if (act) {
+        int nbr[6];
+        periodic_neighbors((int)cell, a.nx, a.ny, a.nz, nbr);
+        T uf[6], ui[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          const int d = f / 2, s = f % 2;
+          const T *nb = src + (long long)nbr[f] * N3;
+          const int lf = s ? 0 : p, li = s ? 1 : p - 1;
+          const int of = d == 0 ? face_dof<N, 0>(lf, l) : (d == 1 ? face_dof<N, 1>(lf, l) : face_dof<N, 2>(lf, l));
+          const int oi = d == 0 ? face_dof<N, 0>(li, l) : (d == 1 ? face_dof<N, 1>(li, l) : face_dof<N, 2>(li, l));
+          uf[f] = nb[of];
+          ui[f] = nb[oi];
         }
-        const T uf = src[nb + of], ui = src[nb + oi];
-        const T hd = ctab(a.dt).hd, w0 = s ? -hd : hd;
-        sF[(cc * 12 + f) * N2 + fa] = uf;
-        sF[(cc * 12 + 6 + f) * N2 + fa] = w0 * (ui - uf);
+        const T hd = ctab(a.dt).hd;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          F[f * N2 + l] = uf[f];
+          F[(6 + f) * N2 + l] = (f % 2 ? -hd : hd) * (ui[f] - uf[f]);
+        }
       }
     }
     __syncthreads();

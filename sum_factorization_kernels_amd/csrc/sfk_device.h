@@ -204,18 +204,18 @@ __device__ __forceinline__ int face_dof(int layer, int fa)
   else return layer * N * N + fa;
 }
 
-// periodic neighbour of cell c (x fastest); cell counts fit in 32 bits
-__device__ __forceinline__ long long neighbor_cell(long long c, int nx, int ny, int nz, int dir, int side)
+// the six periodic neighbours of cell ci (x fastest): below / above in x,
+// then y, then z (cell counts fit in 32 bits)
+__device__ __forceinline__ void periodic_neighbors(int ci, int nx, int ny, int nz, int (&nbr)[6])
 {
-  const int ci = (int)c;
-  int ix = ci % nx;
-  const int r = ci / nx;
-  int iy = r % ny;
-  int iz = r / ny;
-  if (dir == 0) ix = side ? (ix + 1 == nx ? 0 : ix + 1) : (ix == 0 ? nx - 1 : ix - 1);
-  else if (dir == 1) iy = side ? (iy + 1 == ny ? 0 : iy + 1) : (iy == 0 ? ny - 1 : iy - 1);
-  else iz = side ? (iz + 1 == nz ? 0 : iz + 1) : (iz == 0 ? nz - 1 : iz - 1);
-  return ((long long)iz * ny + iy) * nx + ix;
+  const int ix = ci % nx, r = ci / nx, iy = r % ny, iz = r / ny;
+  const int sy = nx, sz = nx * ny;
+  nbr[0] = ci + (ix == 0 ? nx - 1 : -1);
+  nbr[1] = ci + (ix == nx - 1 ? 1 - nx : 1);
+  nbr[2] = ci + (iy == 0 ? (ny - 1) * sy : -sy);
+  nbr[3] = ci + (iy == ny - 1 ? -(ny - 1) * sy : sy);
+  nbr[4] = ci + (iz == 0 ? (nz - 1) * sz : -sz);
+  nbr[5] = ci + (iz == nz - 1 ? -(nz - 1) * sz : sz);
 }
 
 // Workgroups of a launch are dispatched round-robin over the chip's 8 XCDs

@@ -90,11 +90,11 @@ __device__ __forceinline__ void pc_couple(T (&o)[N], const T (&nb)[4], T v1, T v
 }
 
 template <int N, int DIR, typename T>
-__device__ __forceinline__ void pc_load_layers(const T *src, long long cell, int l, int nx, int ny, int nz, T (&nb)[4])
+__device__ __forceinline__ void pc_load_layers(const T *src, const int (&nbr)[6], int l, T (&nb)[4])
 {
   constexpr long long N3 = N * N * N;
-  const T *lo = src + neighbor_cell(cell, nx, ny, nz, DIR, 0) * N3;
-  const T *hi = src + neighbor_cell(cell, nx, ny, nz, DIR, 1) * N3;
+  const T *lo = src + nbr[2 * DIR] * N3;
+  const T *hi = src + nbr[2 * DIR + 1] * N3;
   nb[0] = lo[pc_idx<N, DIR>(l, N - 1)];
   nb[1] = lo[pc_idx<N, DIR>(l, N - 2)];
   nb[2] = hi[pc_idx<N, DIR>(l, 0)];
@@ -121,9 +121,11 @@ precomp_kernel(PcArgs<N, T> a)
 
   T nb[3][4];
   if (COUPLE && act) {
-    pc_load_layers<N, 0>(src, cell, l, a.nx, a.ny, a.nz, nb[0]);
-    pc_load_layers<N, 1>(src, cell, l, a.nx, a.ny, a.nz, nb[1]);
-    pc_load_layers<N, 2>(src, cell, l, a.nx, a.ny, a.nz, nb[2]);
+    int nbr[6];
+    periodic_neighbors((int)cell, a.nx, a.ny, a.nz, nbr);
+    pc_load_layers<N, 0>(src, nbr, l, nb[0]);
+    pc_load_layers<N, 1>(src, nbr, l, nb[1]);
+    pc_load_layers<N, 2>(src, nbr, l, nb[2]);
   }
   // blockDim.x >= cb n^2, so n rounds cover the cb n^3 values of the block
   T keep[CHEBY ? N : 1];

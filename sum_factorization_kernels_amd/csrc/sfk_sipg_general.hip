@@ -59,7 +59,7 @@ struct GenArgs {
   T *tm;
   T c_np, c_tm;
   const T *cm; // [cell][6][q]
-  const T *fm; // [d][cell][8][fa]
+  const T *fm; // [d][cell][7][fa]
   long long ncells;
   long long nitems; // cells (matvec) or cells * N^3 (diagonal)
   int nx, ny, nz;
@@ -73,11 +73,26 @@ __device__ __forceinline__ int pt(int l, int i)
   return line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
 }
 
+// face metric of the two faces of direction DIR: fmv[0] of the lower face
+// (stored with the lower neighbour `lower`), fmv[1] of the upper face
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void load_face_metric(const GenArgs<N, T> &a, int lower, long long cell, int l,
+                                                 T (&fmv)[2][7])
+{
+  constexpr int N2 = N * N;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const T *fmp = a.fm + ((long long)DIR * a.ncells + (s ? cell : (long long)lower)) * 7 * N2 + l;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) fmv[s][e] = fmp[e * N2];
+  }
+}
+
 // traces, neighbour data and face metric of one direction -> test coefficients
 template <int N, int DIR, typename T, bool DIAG>
 __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, const T *Gx, const T *Gy, const T *Gz,
-                                           const T *F, int l, long long cell, const T (&dlo)[N], const T (&dhi)[N],
-                                           T (&tv)[2], T (&td)[2][3])
+                                           const T *F, int l, const T (&dlo)[N], const T (&dhi)[N],
+                                           const T (&fmv)[2][7], T (&tv)[2], T (&td)[2][3])
 {
   constexpr int N2 = N * N;
   const auto &tb = ctab(a.dt);
@@ -111,15 +126,13 @@ __device__ __forceinline__ void face_terms(const GenArgs<N, T> &a, const T *Q, c
         gp[HI] += dhi[m] * FV[m * N + llo];
       }
     }
-    const long long owner = s ? cell : neighbor_cell(cell, a.nx, a.ny, a.nz, DIR, 0);
-    const T *fmp = a.fm + ((long long)DIR * a.ncells + owner) * 7 * N2 + l;
     T ao[3], an[3];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-      ao[e] = s ? fmp[e * N2] : fmp[(3 + e) * N2];
-      an[e] = s ? fmp[(3 + e) * N2] : fmp[e * N2];
+      ao[e] = s ? fmv[s][e] : fmv[s][3 + e];
+      an[e] = s ? fmv[s][3 + e] : fmv[s][e];
     }
-    const T sigma = fmp[6 * N2];
+    const T sigma = fmv[s][6];
     const T sn = s ? T(1) : T(-1);
     T dn = 0;
 #pragma unroll
@@ -166,6 +179,9 @@ sipg_general_kernel(const GenArgs<N, T> a)
   T *Gy = sG[1] + (act ? c : 0) * N3;
   T *Gz = sG[2] + (act ? c : 0) * N3;
   T *F = sF + (!DIAG && act ? c * 12 * N2 : 0);
+  // periodic neighbour cells below / above in x, y, z
+  int nbr[6] = {0, 0, 0, 0, 0, 0};
+  if (act) periodic_neighbors((int)cell, a.nx, a.ny, a.nz, nbr);
 
   // ------------------------------------------------------------------ L
   if (act) {
@@ -183,18 +199,11 @@ sipg_general_kernel(const GenArgs<N, T> a)
     // neighbour layers: thread l of a cell loads face point l of all six
     // faces (twelve independent loads, issued together)
     if (act) {
-      const int ci = (int)cell;
-      const int ix = ci % a.nx, r = ci / a.nx, iy = r % a.ny, iz = r / a.ny;
-      const int sx = 1, sy = a.nx, sz = a.nx * a.ny;
-      // periodic neighbour offsets (in cells) below / above in x, y, z
-      const int off[6] = {ix == 0 ? (a.nx - 1) * sx : -sx, ix == a.nx - 1 ? -(a.nx - 1) * sx : sx,
-                          iy == 0 ? (a.ny - 1) * sy : -sy, iy == a.ny - 1 ? -(a.ny - 1) * sy : sy,
-                          iz == 0 ? (a.nz - 1) * sz : -sz, iz == a.nz - 1 ? -(a.nz - 1) * sz : sz};
       T uf[6], ui[6];
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
         const int d = f / 2, s = f % 2;
-        const T *nb = a.src + (long long)(ci + off[f]) * N3;
+        const T *nb = a.src + (long long)nbr[f] * N3;
         const int lf = s ? 0 : p, li = s ? 1 : p - 1;
         const int of = d == 0 ? face_dof<N, 0>(lf, l) : (d == 1 ? face_dof<N, 1>(lf, l) : face_dof<N, 2>(lf, l));
         const int oi = d == 0 ? face_dof<N, 0>(li, l) : (d == 1 ? face_dof<N, 1>(li, l) : face_dof<N, 2>(li, l));
@@ -279,28 +288,51 @@ sipg_general_kernel(const GenArgs<N, T> a)
         dhi[m] = dtb.Dd[m][l / N];
       }
     }
-    face_terms<N, 0, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, dlo, dhi, tvx, tdx);
-    face_terms<N, 1, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, dlo, dhi, tvy, tdy);
-    face_terms<N, 2, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, cell, dlo, dhi, tvz, tdz);
+    // (loading the face metric of the next direction ahead, or of all
+    // three, measured no faster: the registers cost occupancy)
+    T fmv[2][7];
+    load_face_metric<N, 0>(a, nbr[0], cell, l, fmv);
+    face_terms<N, 0, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, dlo, dhi, fmv, tvx, tdx);
+    load_face_metric<N, 1>(a, nbr[2], cell, l, fmv);
+    face_terms<N, 1, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, dlo, dhi, fmv, tvy, tdy);
+    load_face_metric<N, 2>(a, nbr[4], cell, l, fmv);
+    face_terms<N, 2, T, DIAG>(a, Q, Gx, Gy, Gz, F, l, dlo, dhi, fmv, tvz, tdz);
   }
   __syncthreads();
   // ------------------------------------------------------------------ M
   if (act) {
+    // the metric of this z line in two batches of 3 N loads, each issued
+    // together (two memory round trips per wave instead of N; all 6 N at
+    // once would cost a wave per SIMD in registers)
+    const T *m = a.cm + cell * 6 * N3 + l;
+    T gm[3][N];
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int k = 0; k < N; ++k) gm[e][k] = load_stream(m + e * N3 + k * N2);
     const auto &tb = ctab(a.dt);
     T gx[N], gy[N], gz[N];
     lds_read_line<N, N, 2>(Gx, l, gx);
     lds_read_line<N, N, 2>(Gy, l, gy);
     lds_read_line<N, N, 2>(Gz, l, gz);
-    const T *m = a.cm + cell * 6 * N3;
+    T hx[N], hy[N], hz[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      const int q = k * N2 + l;
-      const T g00 = load_stream(m + q), g11 = load_stream(m + N3 + q), g22 = load_stream(m + 2 * N3 + q),
-              g01 = load_stream(m + 3 * N3 + q), g02 = load_stream(m + 4 * N3 + q), g12 = load_stream(m + 5 * N3 + q);
+      hx[k] = gm[0][k] * gx[k];
+      hy[k] = gm[1][k] * gy[k];
+      hz[k] = gm[2][k] * gz[k];
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int k = 0; k < N; ++k) gm[e][k] = load_stream(m + (3 + e) * N3 + k * N2);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const T g01 = gm[0][k], g02 = gm[1][k], g12 = gm[2][k];
       const T x = gx[k], y = gy[k], z = gz[k];
-      gx[k] = g00 * x + g01 * y + g02 * z + tb.fv0[k] * tdz[0][0] + tb.fv1[k] * tdz[1][0];
-      gy[k] = g01 * x + g11 * y + g12 * z + tb.fv0[k] * tdz[0][1] + tb.fv1[k] * tdz[1][1];
-      gz[k] = g02 * x + g12 * y + g22 * z + tb.fv0[k] * tdz[0][2] + tb.fv1[k] * tdz[1][2];
+      gx[k] = hx[k] + g01 * y + g02 * z + tb.fv0[k] * tdz[0][0] + tb.fv1[k] * tdz[1][0];
+      gy[k] = g01 * x + hy[k] + g12 * z + tb.fv0[k] * tdz[0][1] + tb.fv1[k] * tdz[1][1];
+      gz[k] = g02 * x + g12 * y + hz[k] + tb.fv0[k] * tdz[0][2] + tb.fv1[k] * tdz[1][2];
     }
     lds_write_line<N, N, 2>(Gx, l, gx);
     lds_write_line<N, N, 2>(Gy, l, gy);
