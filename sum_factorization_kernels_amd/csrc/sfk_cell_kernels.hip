@@ -129,44 +129,75 @@ __device__ __forceinline__ void grad_phase(const CellArgs<N, T> &a, const T *Ace
   if constexpr (MODE == Mode::DG) sipg_faces<N, DIR, T>(a, q, F, l, r);
 }
 
-// Colored store of one x-line of a continuous cell.  A workgroup owns CB
-// consecutive cells of one x row (a "segment"); the units (row y, row z,
-// segment) are coloured by the parity of each index, so two units of one
-// colour share no node and a launch writes every node at most once.  The
-// colours are launched in ascending order; for every node the unit with the
-// smallest colour among those that touch it stores (Dirichlet nodes: 0) and
-// later ones add, which makes a zero fill and atomics unnecessary.  Inside a
-// segment the first x-plane of cell c+1 (left in LDS by its threads) is
-// added to the last x-plane of cell c.
+// Colored store of a continuous segment.  A workgroup owns CB consecutive
+// cells of one x row (a "segment"); the units (row y, row z, segment) are
+// coloured by the parity of each index, so two units of one colour share no
+// node and a launch writes every node at most once.  The colours are
+// launched in ascending order; for every node the unit with the smallest
+// colour among those that touch it stores (Dirichlet nodes: 0) and later
+// ones add, which makes a zero fill and atomics unnecessary.
+//
+// The segment's output is (p+1)^2 node rows of ncs*p+1 consecutive nodes
+// (ncs cells).  The threads' x lines are merged into those rows in LDS (the
+// first x-plane of cell c+1, left in sB by its threads, is added to the last
+// x-plane of cell c), and the rows are then written with consecutive lanes
+// on consecutive nodes: a store instruction covers a few cache lines
+// instead of one row per lane.  Values added to are all loaded before the
+// first store.  Called by every thread of the workgroup (barrier inside);
+// sA (the A sweep array, dead at this point) holds the rows.
 template <int N, typename T>
-__device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&o)[N], const T *sB, int c, int l,
-                                                   int cx, int cy, int cz, int seg, int cx0, int cx_end, T *dst)
+__device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&o)[N], T *sA, const T *sB, bool act,
+                                                   int c, int l, int cy, int cz, int seg, int cx0, int cx_end, T *dst)
 {
-  constexpr int p = N - 1, N3 = N * N * N;
-  const int j = l % N, k = l / N;
-  if (cx + 1 < cx_end) o[p] += sB[(c + 1) * N3 + l];
-  const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
-  const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;
-  // smallest colour of the units covering rows [ylo,yhi] x [zlo,zhi] x
-  // segments [slo,shi] (each range has one or two entries)
-  auto min_color = [&](int slo, int shi) {
-    const int py = ylo == yhi ? (ylo & 1) : 0, pz = zlo == zhi ? (zlo & 1) : 0, ps = slo == shi ? (slo & 1) : 0;
-    return py + 2 * pz + 4 * ps;
-  };
-  const int m_mid = min_color(seg, seg);
-  const int Y = cy * p + j, Z = cz * p + k;
-  const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
-  const long long g0 = ((long long)Z * a.gy + Y) * a.gx + (long long)cx * p;
+  constexpr int p = N - 1, N2 = N * N, N3 = N2 * N;
+  const int ncs = cx_end - cx0, Lx = ncs * p + 1;
+  if (act) {
+    if (c + 1 < ncs) o[p] += sB[(c + 1) * N3 + l];
+    T *row = sA + l * Lx + c * p;
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    if (i == 0 && cx > cx0) continue; // added into the previous cell's plane
-    int m = m_mid;
-    if (i == 0 && seg > 0) m = min_color(seg - 1, seg);
-    if (i == p && cx + 1 == cx_end && seg + 1 < a.nseg) m = min_color(seg, seg + 1);
-    const int X = cx * p + i;
-    const bool bd = byz || X == 0 || X == a.gx - 1;
-    if (m == a.color) dst[g0 + i] = bd ? T(0) : o[i];
-    else if (!bd) dst[g0 + i] += o[i];
+    for (int i = 0; i < N; ++i)
+      if (i > 0 || c == 0) row[i] = o[i];
+  }
+  __syncthreads();
+  // blockDim.x >= CB (p+1)^2 and N2 (ncs p + 1) <= CB N3: N rounds cover the rows
+  const int total = N2 * Lx;
+  int mode[N]; // 0 nothing, 1 store, 2 add
+  long long gi[N];
+  T val[N], old[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    const int idx = (int)threadIdx.x + r * (int)blockDim.x;
+    mode[r] = 0;
+    gi[r] = 0;
+    val[r] = T(0);
+    old[r] = T(0);
+    if (idx < total) {
+      const int rw = idx / Lx, xi = idx - rw * Lx;
+      const int j = rw % N, k = rw / N;
+      const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
+      const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;
+      // smallest colour of the units covering rows [ylo,yhi] x [zlo,zhi] x
+      // segments [slo,shi] (each range has one or two entries)
+      const int py = ylo == yhi ? (ylo & 1) : 0, pz = zlo == zhi ? (zlo & 1) : 0;
+      int ps = seg & 1;
+      if ((xi == 0 && seg > 0) || (xi == Lx - 1 && seg + 1 < a.nseg)) ps = 0;
+      const int m = py + 2 * pz + 4 * ps;
+      const int X = cx0 * p + xi, Y = cy * p + j, Z = cz * p + k;
+      const bool bd = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
+      gi[r] = ((long long)Z * a.gy + Y) * a.gx + X;
+      val[r] = bd ? T(0) : sA[idx];
+      if (m == a.color) {
+        mode[r] = 1;
+      } else if (!bd) {
+        mode[r] = 2;
+        old[r] = dst[gi[r]];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (mode[r] == 1) dst[gi[r]] = val[r];
+    else if (mode[r] == 2) dst[gi[r]] = old[r] + val[r];
   }
 }
 
@@ -225,6 +256,8 @@ cell_kernel(const CellArgs<N, T> a)
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
       if constexpr (MODE == Mode::Cont) {
+        // (staging the segment's node rows through LDS for coalesced loads
+        // measured slower: the extra barrier costs more than it saves)
         const int X = ccx * p + l % N, Y = ccy * p + l / N;
         const bool bxy = X == 0 || Y == 0 || X == a.gx - 1 || Y == a.gy - 1;
         const long long g0 = (long long)ccz * p * a.gy * a.gx + (long long)Y * a.gx + X;
@@ -339,7 +372,20 @@ cell_kernel(const CellArgs<N, T> a)
     }
     __syncthreads();
     // --------------------------------------------------------------- By
+    // DG Chebyshev: this thread's x line of rhs, diag, tmp and the input,
+    // loaded one phase ahead of the update that uses them
+    T crhs[CHEBY ? N : 1], cdiag[CHEBY ? N : 1], ctm[CHEBY ? N : 1], csrc[CHEBY ? N : 1];
     if (act) {
+      if constexpr (MODE == Mode::DG && CHEBY) {
+        const long long g0 = cell * N3 + l * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          crhs[i] = a.rhs[g0 + i];
+          cdiag[i] = a.diag[g0 + i];
+          ctm[i] = a.tm[g0 + i];
+          csrc[i] = src[g0 + i];
+        }
+      }
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
       lds_read_line<N, N, 1>(A, l, u);
@@ -365,7 +411,7 @@ cell_kernel(const CellArgs<N, T> a)
     if constexpr (MODE == Mode::Cont) {
       if (a.color >= 0) {
         __syncthreads();
-        if (act) cont_store_colored<N, T>(a, o, sB, c, l, ccx, ccy, ccz, seg, cx0, cx_end, dst);
+        cont_store_colored<N, T>(a, o, sA, sB, act, c, l, ccy, ccz, seg, cx0, cx_end, dst);
       } else if (act) {
         const int Y = ccy * p + l % N, Z = ccz * p + l / N;
         const bool byz = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
@@ -387,10 +433,10 @@ cell_kernel(const CellArgs<N, T> a)
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           const long long g = g0 + i;
-          const T res = o[i] - a.rhs[g];
-          const T tmv = a.c_tm * a.tm[g] + a.c_np * a.diag[g] * res;
+          const T res = o[i] - crhs[i];
+          const T tmv = a.c_tm * ctm[i] + a.c_np * cdiag[i] * res;
           a.tm[g] = tmv;
-          dst[g] = src[g] - tmv;
+          dst[g] = csrc[i] - tmv;
         }
       } else {
         const long long g0 = cell * N3 + l * N;

@@ -390,7 +390,20 @@ sipg_general_kernel(const GenArgs<N, T> a)
   }
   __syncthreads();
   // ----------------------------------------------------------------- Sy
+  // Chebyshev: this thread's x line of rhs, diag, tmp and the input, loaded
+  // one phase ahead of the update that uses them
+  T crhs[CHEBY ? N : 1], cdiag[CHEBY ? N : 1], ctm[CHEBY ? N : 1], csrc[CHEBY ? N : 1];
   if (act) {
+    if constexpr (CHEBY) {
+      const long long g0 = cell * N3 + l * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        crhs[i] = a.rhs[g0 + i];
+        cdiag[i] = a.diag[g0 + i];
+        ctm[i] = a.tm[g0 + i];
+        csrc[i] = a.src[g0 + i];
+      }
+    }
     const auto &tb = ctab(a.dt);
     T u[N], v[N];
     lds_read_line<N, N, 1>(Q, l, u);
@@ -412,10 +425,10 @@ sipg_general_kernel(const GenArgs<N, T> a)
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           const long long g = g0 + i;
-          const T res = o[i] - a.rhs[g];
-          const T tmv = a.c_tm * a.tm[g] + a.c_np * a.diag[g] * res;
+          const T res = o[i] - crhs[i];
+          const T tmv = a.c_tm * ctm[i] + a.c_np * cdiag[i] * res;
           a.tm[g] = tmv;
-          a.dst[g] = a.src[g] - tmv;
+          a.dst[g] = csrc[i] - tmv;
         }
       } else {
 #pragma unroll
