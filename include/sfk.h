@@ -178,7 +178,9 @@ sfk_status sfk_op_rhs_constant(sfk_op *op, double f, void *b, void *stream);
  * run-dependent order). */
 typedef struct sfk_schedule {
   int cells_per_block; /* cells per workgroup; 0 = the degree's default */
-  int block_order;     /* 0 linear, 1 one contiguous range of cells per XCD */
+  int block_order;     /* 0 linear, 1 one contiguous range of cells per XCD,
+                          2 as 1 with the DG kernels walking slabs of 4
+                          z-layers x, z, y (neighbour data re-read from L2) */
   int cont_scatter;    /* continuous Laplacian: -1 automatic, 0 zero fill +
                           atomic adds, 1 colored stores (8 launches) */
 } sfk_schedule;

@@ -166,7 +166,8 @@ advect_kernel(const AdvArgs<N, T> a)
   T *sB = sA + CB * N3;
   T *sF = sB + CB * N3;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
-  const long long cell0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB, cell = cell0 + c;
+  const long long cell0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB;
+  const long long cell = traverse_cell(cell0 + c, a.nx, a.ny, a.nz, a.order);
   const bool act = c < CB && cell < a.ncells;
   T *A = sA + (act ? c : 0) * N3;
   T *B = sB + (act ? c : 0) * N3;

@@ -191,6 +191,9 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
 
   sfk_op *op = new sfk_op();
   op->d = d;
+  // per-q SIPG: slab traversal by default (its re-read neighbour layers and
+  // face metric stay in L2: HBM traffic 1.18x -> 1.07x the algorithmic bytes)
+  if (d.kind == SFK_DG_SIPG && d.geometry == SFK_GEOM_PER_Q) op->sched_order = 2;
   if (!build_tables(d.degree, d.basis, op->tab)) {
     delete op;
     set_error("table construction failed");

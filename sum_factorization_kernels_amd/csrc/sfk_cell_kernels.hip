@@ -240,7 +240,7 @@ cell_kernel(const CellArgs<N, T> a)
       ccz = ci / a.nx / a.ny;
     }
   }
-  const long long cell = cell0 + c;
+  const long long cell = MODE == Mode::DG ? traverse_cell(cell0 + c, a.nx, a.ny, a.nz, a.order) : cell0 + c;
   T *A = sA + (act ? c : 0) * N3;
   T *B = sB + (act ? c : 0) * N3;
   T *F = sF + (FACES && act ? c * 12 * N2 : 0);

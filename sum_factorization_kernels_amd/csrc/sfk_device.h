@@ -232,6 +232,26 @@ __device__ __forceinline__ unsigned logical_block(unsigned b, unsigned nb, int o
   return x * q + (x < r ? x : r) + k;
 }
 
+// Cell at position q of a DG kernel's traversal.  Block order 2 walks slabs
+// of kSlab z-layers x fastest, then z, then y: a cell's lower z and y
+// neighbours (whose face layers and face metric it re-reads) were processed
+// nx and nx*kSlab positions earlier, by the same XCD, and are still in its
+// L2 (in index order the lower z neighbour is nx*ny positions back).
+// Orders 0 and 1 walk the cells in index order.  A bijection of [0, ncells).
+constexpr int kSlab = 4;
+__device__ __forceinline__ long long traverse_cell(long long q, int nx, int ny, int nz, int order)
+{
+  if (order != 2) return q;
+  const long long layer = (long long)nx * ny;
+  const int z0 = (int)(q / (layer * kSlab)) * kSlab;
+  const int zs = nz - z0 < kSlab ? nz - z0 : kSlab;
+  const long long r = q - (long long)z0 * layer;
+  const int x = (int)(r % nx);
+  const long long r2 = r / nx;
+  const int zz = (int)(r2 % zs), y = (int)(r2 / zs);
+  return ((long long)(z0 + zz) * ny + y) * nx + x;
+}
+
 // Waves per SIMD that a kernel's LDS allows at its default cells per
 // workgroup (gfx950: 160 KiB of LDS per CU, 4 SIMDs, at most 8 waves per
 // SIMD).  The operator kernels size their LDS at run time (dynamic LDS), so

@@ -172,7 +172,7 @@ sipg_general_kernel(const GenArgs<N, T> a)
   const int t = threadIdx.x, c = t / N2, l = t % N2;
   const long long item0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB, item = item0 + c;
   const bool act = c < CB && item < a.nitems;
-  const long long cell = DIAG ? item / N3 : item;
+  const long long cell = DIAG ? item / N3 : traverse_cell(item, a.nx, a.ny, a.nz, a.order);
   const int unit = DIAG ? (int)(item % N3) : 0; // diagonal: dof of the unit vector
   T *Q = sQ + (act ? c : 0) * N3;
   T *Gx = sG[0] + (act ? c : 0) * N3;

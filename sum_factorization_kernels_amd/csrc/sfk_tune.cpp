@@ -91,7 +91,7 @@ sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s)
   const int m = max_cb(op);
   if (s->cells_per_block < 0 || s->cells_per_block > m)
     TUNE_FAIL(SFK_ERR_INVALID_ARG, "cells_per_block must be in 0.." + std::to_string(m));
-  if (s->block_order != 0 && s->block_order != 1) TUNE_FAIL(SFK_ERR_INVALID_ARG, "block_order must be 0 or 1");
+  if (s->block_order < 0 || s->block_order > 2) TUNE_FAIL(SFK_ERR_INVALID_ARG, "block_order must be 0, 1 or 2");
   if (s->cont_scatter < -1 || s->cont_scatter > 1) TUNE_FAIL(SFK_ERR_INVALID_ARG, "cont_scatter must be -1, 0 or 1");
   op->sched_cb = s->cells_per_block;
   op->sched_order = s->block_order;
@@ -124,9 +124,11 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
   if (status == SFK_OK) {
     const int m = max_cb(op);
     std::vector<int> scatters = {op->cont_scatter};
+    // order 2 (slab traversal) exists in the DG kernels only
+    const int max_order = op->d.kind == SFK_DG_SIPG || op->d.kind == SFK_DG_ADVECT ? 2 : 1;
     if (op->d.kind == SFK_CONT_LAPLACE) scatters = {0, 1};
     for (int sc : scatters)
-      for (int order = 0; order <= 1 && status == SFK_OK; ++order)
+      for (int order = 0; order <= max_order && status == SFK_OK; ++order)
         for (int cb = 1; cb <= m && status == SFK_OK; ++cb) {
           op->sched_cb = cb;
           op->sched_order = order;

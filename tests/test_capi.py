@@ -101,12 +101,13 @@ def test_schedule_defaults_and_validation(degree, kind, geom, default_cb):
     if kind == sfk.DG_ADVECT:
         kw["velocity"] = (1.0, 0.15, -0.05)
     op = sfk.Operator(kind, degree, (4, 4, 4), geometry=geom, inv_jac=(4, 4, 4), **kw)
-    assert op.schedule == (default_cb, 1, -1)
+    # the per-q SIPG kernel walks slabs by default (block order 2)
+    assert op.schedule == (default_cb, 2 if geom == sfk.GEOM_PER_Q else 1, -1)
     m = op.max_cells_per_block
     assert default_cb <= m <= 256 // (degree + 1) ** 2
     op.set_schedule(1, 0, 1)
     assert op.schedule == (1, 0, 1)
-    for bad in [(m + 1, 1, -1), (-1, 1, -1), (0, 2, -1), (0, 1, 2)]:
+    for bad in [(m + 1, 1, -1), (-1, 1, -1), (0, 3, -1), (0, 1, 2)]:
         with pytest.raises(sfk.SfkError):
             op.set_schedule(*bad)
     with pytest.raises(sfk.SfkError):
