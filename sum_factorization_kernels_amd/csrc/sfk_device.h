@@ -242,14 +242,14 @@ constexpr int kSlab = 4;
 __device__ __forceinline__ long long traverse_cell(long long q, int nx, int ny, int nz, int order)
 {
   if (order != 2) return q;
-  const long long layer = (long long)nx * ny;
-  const int z0 = (int)(q / (layer * kSlab)) * kSlab;
-  const int zs = nz - z0 < kSlab ? nz - z0 : kSlab;
-  const long long r = q - (long long)z0 * layer;
-  const int x = (int)(r % nx);
-  const long long r2 = r / nx;
-  const int zz = (int)(r2 % zs), y = (int)(r2 / zs);
-  return ((long long)(z0 + zz) * ny + y) * nx + x;
+  // cell counts fit in 31 bits: 32-bit unsigned arithmetic throughout
+  const unsigned uq = (unsigned)q, unx = (unsigned)nx, layer = unx * (unsigned)ny;
+  const unsigned z0 = uq / layer / kSlab * kSlab;
+  const unsigned zs = (unsigned)nz - z0 < (unsigned)kSlab ? (unsigned)nz - z0 : (unsigned)kSlab;
+  const unsigned r = uq - z0 * layer;
+  const unsigned r2 = r / unx;
+  const unsigned x = r - r2 * unx, y = r2 / zs, zz = r2 - y * zs;
+  return (long long)(((z0 + zz) * (unsigned)ny + y) * unx + x);
 }
 
 // Waves per SIMD that a kernel's LDS allows at its default cells per
