@@ -265,10 +265,17 @@ def main():
         del x3f, y3f
         bn = float(torch.linalg.norm(b3))
         try:
-            x3.zero_()
-            st_m = sfk.solve_cg_cheby(o3, s32, b3, x3, 1e-10, 2000, 6, 12)
-            x3.zero_()
-            st_d = sfk.solve_cg_cheby(o3, o3, b3, x3, 1e-10, 2000, 6, 12)
+            # both solvers at Chebyshev degree 6, and each at its own best
+            # degree of {4, 6, 8} (the degree trades outer FP64 iterations
+            # against inner smoother steps differently for the two)
+            runs = {}
+            for deg in (4, 6, 8):
+                for name, sm in (("mixed", s32), ("fp64", o3)):
+                    x3.zero_()
+                    runs[(name, deg)] = sfk.solve_cg_cheby(o3, sm, b3, x3, 1e-10, 3000, deg, 12)
+            st_m, st_d = runs[("mixed", 6)], runs[("fp64", 6)]
+            best_m = min((st.seconds, deg) for (name, deg), st in runs.items() if name == "mixed")
+            best_d = min((st.seconds, deg) for (name, deg), st in runs.items() if name == "fp64")
             extra["cg_cheby_q6_64cube"] = {
                 "mixed_seconds": round(st_m.seconds, 4), "fp64_seconds": round(st_d.seconds, 4),
                 "speedup": round(st_d.seconds / st_m.seconds, 3),
@@ -276,9 +283,14 @@ def main():
                 "mixed_iterations": st_m.iterations, "fp64_iterations": st_d.iterations,
                 "mixed_matvecs_fp64_fp32": [st_m.matvecs_outer, st_m.matvecs_inner],
                 "fp64_matvecs": st_d.matvecs_outer + st_d.matvecs_inner,
-                "lambda_max": round(st_d.lambda_max, 6)}
-            log(f"CG-Chebyshev Q6 64^3: mixed {st_m.seconds:.3f} s ({st_m.iterations} it), "
-                f"FP64 {st_d.seconds:.3f} s ({st_d.iterations} it)")
+                "lambda_max": round(st_d.lambda_max, 6),
+                "seconds_by_degree": {f"{name}_deg{deg}": round(st.seconds, 4) for (name, deg), st in runs.items()},
+                "best_mixed": {"degree": best_m[1], "seconds": round(best_m[0], 4)},
+                "best_fp64": {"degree": best_d[1], "seconds": round(best_d[0], 4)},
+                "speedup_best_vs_best": round(best_d[0] / best_m[0], 3)}
+            log(f"CG-Chebyshev Q6 64^3 (degree 6): mixed {st_m.seconds:.3f} s ({st_m.iterations} it), "
+                f"FP64 {st_d.seconds:.3f} s ({st_d.iterations} it); best: mixed {best_m[0]:.3f} s (degree "
+                f"{best_m[1]}), FP64 {best_d[0]:.3f} s (degree {best_d[1]})")
         except sfk.SfkError as e:
             extra["cg_cheby_q6_64cube"] = {"error": str(e)}
             log(f"CG-Chebyshev Q6 64^3 failed: {e}")
