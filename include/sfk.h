@@ -151,7 +151,7 @@ sfk_status sfk_rk_stage(sfk_op *op, const void *src, const void *upd, void *vec_
 /* diag = diagonal of A (point-Jacobi); continuous Laplacian: 0 on Dirichlet rows */
 sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream);
 
-/* Gradients (no reference counterpart; north-star extension):
+/* Gradients (no reference counterpart):
  * advection: with r = A u - f, J = 0.5 |r|^2, writes dJ/d(velocity) into
  *   grad_cell [cell][3][q] and grad_face [d][cell][q] (layouts of
  *   sfk_op_set_velocity); lambda = A u - f is passed in by the caller;

@@ -4,7 +4,7 @@
 
 Compiles every source under csrc/ with hipcc for gfx950 into objects under
 csrc/_build/ and links sum_factorization_kernels_amd/libsfk.so.  Always
-recompiles (the driver's "does it build" check).
+recompiles every source (a build call doubles as a compile check).
 """
 import argparse
 import concurrent.futures

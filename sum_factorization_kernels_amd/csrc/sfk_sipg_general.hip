@@ -67,12 +67,6 @@ struct GenArgs {
   int order; // block order (logical_block)
 };
 
-template <int N, int DIR>
-__device__ __forceinline__ int pt(int l, int i)
-{
-  return line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
-}
-
 // face metric of the two faces of direction DIR: fmv[0] of the lower face
 // (stored with the lower neighbour `lower`), fmv[1] of the upper face
 template <int N, int DIR, typename T>

@@ -1,5 +1,5 @@
 // Chebyshev-preconditioned CG for the continuous Q_p Laplacian, all on the
-// device (the reference has no solver; north-star extension).
+// device (the reference has no solver).
 //
 // PCG in FP64 on `op` (iterate, residual, search direction, inner products),
 // preconditioned by a Chebyshev(k) point-Jacobi polynomial applied on the
