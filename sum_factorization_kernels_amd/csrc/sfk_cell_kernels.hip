@@ -186,14 +186,15 @@ __device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&
       const bool bd = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
       gi[r] = ((long long)Z * a.gy + Y) * a.gx + X;
       val[r] = bd ? T(0) : sA[idx];
-      if (m == a.color) {
-        mode[r] = 1;
-      } else if (!bd) {
-        mode[r] = 2;
-        old[r] = dst[gi[r]];
-      }
+      mode[r] = m == a.color ? 1 : (bd ? 0 : 2);
     }
   }
+  // the loads in a loop of their own, so that they are all in flight
+  // together (interleaved with the index arithmetic above, the compiler
+  // waited for each before the next)
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+    if (mode[r] == 2) old[r] = dst[gi[r]];
 #pragma unroll
   for (int r = 0; r < N; ++r) {
     if (mode[r] == 1) dst[gi[r]] = val[r];

@@ -127,16 +127,19 @@ precomp_kernel(PcArgs<N, T> a)
     pc_load_layers<N, 1>(src, nbr, l, nb[1]);
     pc_load_layers<N, 2>(src, nbr, l, nb[2]);
   }
-  // blockDim.x >= cb n^2, so n rounds cover the cb n^3 values of the block
-  T keep[CHEBY ? N : 1];
+  // blockDim.x >= cb n^2, so n rounds cover the cb n^3 values of the block;
+  // the loads are branch-free (index clamped into the block) so that all n
+  // are in flight together, and only the LDS writes are guarded
+  T keep[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int i = t + k * (int)blockDim.x;
-    if (i < nval) {
-      const T v = sb[i];
-      U[i] = v;
-      if constexpr (CHEBY) keep[k] = v;
-    }
+    keep[k] = sb[i < nval ? i : nval - 1];
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int i = t + k * (int)blockDim.x;
+    if (i < nval) U[i] = keep[k];
   }
   __syncthreads();
 
