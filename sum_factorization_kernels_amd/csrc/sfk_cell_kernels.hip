@@ -161,6 +161,10 @@ __device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&
   __syncthreads();
   // blockDim.x >= CB (p+1)^2 and N2 (ncs p + 1) <= CB N3: N rounds cover the rows
   const int total = N2 * Lx;
+  // idx / Lx as (idx * mlx) >> 20 with mlx = ceil(2^20 / Lx): exact while
+  // idx * (mlx - 2^20 / Lx) < 2^20 / Lx, i.e. for idx < 2^20 / Lx, which
+  // holds here (idx < N blockDim.x <= 2304, Lx <= 8 * 64 + 1)
+  const unsigned mlx = ((1u << 20) + unsigned(Lx) - 1u) / unsigned(Lx);
   int mode[N]; // 0 nothing, 1 store, 2 add
   long long gi[N];
   T val[N], old[N];
@@ -172,7 +176,7 @@ __device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&
     val[r] = T(0);
     old[r] = T(0);
     if (idx < total) {
-      const int rw = idx / Lx, xi = idx - rw * Lx;
+      const int rw = (int)((unsigned(idx) * mlx) >> 20), xi = idx - rw * Lx;
       const int j = rw % N, k = rw / N;
       const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
       const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;

@@ -253,14 +253,19 @@ __device__ __forceinline__ long long traverse_cell(long long q, int nx, int ny, 
 }
 
 // Waves per SIMD that a kernel's LDS allows at its default cells per
-// workgroup (gfx950: 160 KiB of LDS per CU, 4 SIMDs, at most 8 waves per
-// SIMD).  The operator kernels size their LDS at run time (dynamic LDS), so
-// the compiler cannot see that LDS caps their occupancy; passing this bound
-// as amdgpu_waves_per_eu(1, W) lets it spend up to 512/W registers per lane
-// instead of squeezing them for waves that can never be resident.
+// workgroup (gfx950: 160 KiB of LDS per CU, allocated to workgroups in
+// 1280-byte granules -- measured on MI355X with fixed-duration workgroups:
+// 32000 B per workgroup still fits five per CU, 32256 B only four -- 4
+// SIMDs, at most 8 waves per SIMD).  The operator kernels size their LDS at
+// run time (dynamic LDS), so the compiler cannot see that LDS caps their
+// occupancy; passing this bound as amdgpu_waves_per_eu(1, W) lets it spend
+// up to 512/W registers per lane instead of squeezing them for waves that
+// can never be resident.
+constexpr int kLdsGranule = 1280;
 constexpr int lds_waves_per_simd(int lds_bytes_per_block, int threads_per_block)
 {
-  const int blocks = lds_bytes_per_block > 0 ? 163840 / lds_bytes_per_block : 32;
+  const int alloc = (lds_bytes_per_block + kLdsGranule - 1) / kLdsGranule * kLdsGranule;
+  const int blocks = alloc > 0 ? 163840 / alloc : 32;
   const int waves = blocks * ((threads_per_block + 63) / 64);
   const int w = (waves + 3) / 4;
   return w < 1 ? 1 : (w > 8 ? 8 : w);
