@@ -105,6 +105,8 @@ hipError_t launch_precomp(const PrecompLaunch &L, hipStream_t stream);
 hipError_t launch_precomp_diagonal(const PrecompLaunch &L, hipStream_t stream);
 // the tables of launch_precomp in a new device allocation (f: direction factors)
 hipError_t upload_precomp_tables(int n, const Precomp1D &pc, const double f[3], bool fp32, void **dev);
+// LDS bytes per cell of launch_precomp's kernel (n points per direction)
+size_t precomp_lds_per_cell(int n, bool fp32);
 
 // continuous Laplacian: coefficient gradient g_q = JxW grad u . grad lambda
 struct CoefGradLaunch {

@@ -51,11 +51,52 @@ constexpr int pc_cells_per_block()
   return kPcMaxThreads / (N * N) > 0 ? kPcMaxThreads / (N * N) : 1;
 }
 
+// LDS layout of a cell's values: x fastest, rows of N, planes PP values
+// apart, cells CS apart.  LDS serves 8-byte accesses in passes of 16 lanes
+// and 16-byte accesses in passes of 8 lanes over 32 four-byte banks
+// (measured on MI355X).  For N = 4 in FP64 the unpadded cell (PP = 16)
+// puts the 16 y lines of a cell on 4 bank pairs (4-way conflicts) and the x
+// lines' 16-byte accesses 2-way; PP = 18 spreads the x lines over all 16
+// bank pairs and leaves the y lines 2-way (conflict cycles per wave 448 ->
+// 112 in that model).
+template <int N, typename T>
+struct PcLds {
+  static constexpr bool pad = N == 4 && sizeof(T) == 8;
+  static constexpr int PP = pad ? 18 : N * N;
+  static constexpr int CS = pad ? 3 * 18 + 16 : N * N * N;
+};
+
+// LDS offset of the dof d (x fastest) of a cell
+template <int N, typename T>
+__device__ __forceinline__ int pc_off(int d)
+{
+  return (d / (N * N)) * PcLds<N, T>::PP + d % (N * N);
+}
+
+template <int N, typename T, int DIR>
+__device__ __forceinline__ void pc_read(const T *cell, int l, T (&v)[N])
+{
+  constexpr int PP = PcLds<N, T>::PP;
+  const int b = DIR == 0 ? (l / N) * PP + (l % N) * N : (DIR == 1 ? (l / N) * PP + l % N : l);
+  constexpr int st = DIR == 0 ? 1 : (DIR == 1 ? N : PP);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = cell[b + i * st];
+}
+template <int N, typename T, int DIR>
+__device__ __forceinline__ void pc_write(T *cell, int l, const T (&v)[N])
+{
+  constexpr int PP = PcLds<N, T>::PP;
+  const int b = DIR == 0 ? (l / N) * PP + (l % N) * N : (DIR == 1 ? (l / N) * PP + l % N : l);
+  constexpr int st = DIR == 0 ? 1 : (DIR == 1 ? N : PP);
+#pragma unroll
+  for (int i = 0; i < N; ++i) cell[b + i * st] = v[i];
+}
+
 template <int N, typename T>
 constexpr int pc_waves()
 {
   constexpr int cb = pc_cells_per_block<N>();
-  return lds_waves_per_simd(3 * cb * N * N * N * (int)sizeof(T), cb * N * N);
+  return lds_waves_per_simd(3 * cb * PcLds<N, T>::CS * (int)sizeof(T), cb * N * N);
 }
 
 template <int N, typename T>
@@ -107,8 +148,9 @@ precomp_kernel(PcArgs<N, T> a)
 {
   constexpr int N2 = N * N, N3 = N2 * N;
   const int cb = a.cb;
+  constexpr int CS = PcLds<N, T>::CS;
   T *U = dyn_lds<T>();
-  T *P = U + cb * N3, *Q = P + cb * N3;
+  T *P = U + cb * CS, *Q = P + cb * CS;
   const long long c0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * cb;
   const long long voff = (long long)blockIdx.y * a.nvec;
   const T *src = a.src + voff;
@@ -139,28 +181,28 @@ precomp_kernel(PcArgs<N, T> a)
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int i = t + k * (int)blockDim.x;
-    if (i < nval) U[i] = keep[k];
+    if (i < nval) U[(i / N3) * CS + pc_off<N, T>(i % N3)] = keep[k];
   }
   __syncthreads();
 
-  T *Uc = U + c * N3, *Pc = P + c * N3, *Qc = Q + c * N3;
+  T *Uc = U + c * CS, *Pc = P + c * CS, *Qc = Q + c * CS;
   T r[N];
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    lds_read_line<N, N, 0>(Uc, l, v);
+    pc_read<N, T, 0>(Uc, l, v);
     eo_apply<N, 1>(pt.L, v, o);
     if constexpr (COUPLE) pc_couple(o, nb[0], pt.v1, pt.v2);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] *= pt.f[0];
-    lds_write_line<N, N, 0>(Pc, l, o);
-    lds_read_line<N, N, 1>(Uc, l, v);
+    pc_write<N, T, 0>(Pc, l, o);
+    pc_read<N, T, 1>(Uc, l, v);
     eo_apply<N, 1>(pt.L, v, o);
     if constexpr (COUPLE) pc_couple(o, nb[1], pt.v1, pt.v2);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] *= pt.f[1];
-    lds_write_line<N, N, 1>(Qc, l, o);
-    lds_read_line<N, N, 2>(Uc, l, v);
+    pc_write<N, T, 1>(Qc, l, o);
+    pc_read<N, T, 2>(Uc, l, v);
     eo_apply<N, 1>(pt.L, v, r);
     if constexpr (COUPLE) pc_couple(r, nb[2], pt.v1, pt.v2);
 #pragma unroll
@@ -170,38 +212,38 @@ precomp_kernel(PcArgs<N, T> a)
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    lds_read_line<N, N, 2>(Pc, l, v);
+    pc_read<N, T, 2>(Pc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    lds_write_line<N, N, 2>(Pc, l, o);
-    lds_read_line<N, N, 2>(Qc, l, v);
+    pc_write<N, T, 2>(Pc, l, o);
+    pc_read<N, T, 2>(Qc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    lds_write_line<N, N, 2>(Qc, l, o);
-    lds_write_line<N, N, 2>(Uc, l, r);
+    pc_write<N, T, 2>(Qc, l, o);
+    pc_write<N, T, 2>(Uc, l, r);
   }
   __syncthreads();
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    lds_read_line<N, N, 1>(Pc, l, v);
+    pc_read<N, T, 1>(Pc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    lds_write_line<N, N, 1>(Pc, l, o);
-    lds_read_line<N, N, 1>(Uc, l, v);
+    pc_write<N, T, 1>(Pc, l, o);
+    pc_read<N, T, 1>(Uc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    lds_read_line<N, N, 1>(Qc, l, v);
+    pc_read<N, T, 1>(Qc, l, v);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] = v[i] + o[i];
-    lds_write_line<N, N, 1>(Qc, l, o);
+    pc_write<N, T, 1>(Qc, l, o);
   }
   __syncthreads();
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    lds_read_line<N, N, 0>(Qc, l, v);
+    pc_read<N, T, 0>(Qc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    lds_read_line<N, N, 0>(Pc, l, v);
+    pc_read<N, T, 0>(Pc, l, v);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] = o[i] + v[i];
-    lds_write_line<N, N, 0>(Uc, l, o);
+    pc_write<N, T, 0>(Uc, l, o);
   }
   __syncthreads();
   const long long g0 = voff + c0 * N3;
@@ -209,7 +251,7 @@ precomp_kernel(PcArgs<N, T> a)
   for (int k = 0; k < N; ++k) {
     const int i = t + k * (int)blockDim.x;
     if (i < nval) {
-      const T au = U[i];
+      const T au = U[(i / N3) * CS + pc_off<N, T>(i % N3)];
       if constexpr (CHEBY) {
         // r = A u - rhs; tm = c_tm tm + c_np diag r; dst = u - tm (tm not
         // read when c_tm == 0)
@@ -263,7 +305,7 @@ hipError_t launch_precomp_n(const PrecompLaunch &L, hipStream_t s)
   if (L.cheby && L.nrhs != 1) return hipErrorInvalidValue;
   a.cb = CB;
   a.order = L.order;
-  const size_t lds = size_t(3) * CB * N * N * N * sizeof(T);
+  const size_t lds = size_t(3) * CB * PcLds<N, T>::CS * sizeof(T);
   const dim3 grid((unsigned)((L.ncells + CB - 1) / CB), (unsigned)L.nrhs), block(((CB * N * N + 63) / 64) * 64);
   decltype(&precomp_kernel<N, T, true, false>) kern;
   if (L.couple) kern = L.cheby ? precomp_kernel<N, T, true, true> : precomp_kernel<N, T, true, false>;
@@ -272,6 +314,12 @@ hipError_t launch_precomp_n(const PrecompLaunch &L, hipStream_t s)
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, grid, block, lds, s, a);
   return hipGetLastError();
+}
+
+template <int N, typename T>
+size_t precomp_lds_per_cell_n()
+{
+  return size_t(3) * PcLds<N, T>::CS * sizeof(T);
 }
 
 template <int N, typename T>
@@ -348,6 +396,17 @@ hipError_t launch_precomp_diagonal(const PrecompLaunch &L, hipStream_t s)
 hipError_t upload_precomp_tables(int n, const Precomp1D &pc, const double f[3], bool fp32, void **dev)
 {
   return fp32 ? dispatch_pctab<float>(n, pc, f, dev) : dispatch_pctab<double>(n, pc, f, dev);
+}
+
+size_t precomp_lds_per_cell(int n, bool fp32)
+{
+  switch (n) {
+#define SFK_CASE(NN) \
+  case NN: return fp32 ? precomp_lds_per_cell_n<NN, float>() : precomp_lds_per_cell_n<NN, double>();
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return size_t(3) * n * n * n * (fp32 ? 4 : 8);
+  }
 }
 
 } // namespace sfk
