@@ -28,7 +28,7 @@ template <int N, typename T, int KIND>
 constexpr int adv_waves()
 {
   constexpr int cb = adv_cells_per_block<N>();
-  return lds_waves_per_simd(cb * (2 * N * N * N + (KIND == 2 ? 0 : 6 * N * N)) * (int)sizeof(T), cb * N * N);
+  return lds_waves_per_simd(cb * (2 * CellLds<N, T>::CS + (KIND == 2 ? 0 : 6 * N * N)) * (int)sizeof(T), cb * N * N);
 }
 
 template <int N, typename T>
@@ -114,7 +114,7 @@ __device__ __forceinline__ void adv_dir(const AdvArgs<N, T> &a, const T *Acell, 
 {
   const auto &tb = ctab(a.dt);
   T q[N];
-  lds_read_line<N, N, DIR>(Acell, l, q);
+  cl_read<N, DIR>(Acell, l, q);
   const T c = a.ij[DIR] * a.jxw;
   if constexpr (!ADJ) {
     T g[N];
@@ -161,16 +161,18 @@ advect_kernel(const AdvArgs<N, T> a)
   constexpr int N2 = N * N, N3 = N * N * N;
   constexpr bool ADJ = KIND == 1;
   const int CB = a.cb;
-  // dynamic LDS: A [CB][N3], B [CB][N3], F [CB][6][N2] (not for M^{-1})
+  // dynamic LDS: A [CB][CS], B [CB][CS] (padded cell blocks), F [CB][6][N2]
+  // (not for M^{-1})
+  constexpr int CS = CellLds<N, T>::CS;
   T *sA = dyn_lds<T>();
-  T *sB = sA + CB * N3;
-  T *sF = sB + CB * N3;
+  T *sB = sA + CB * CS;
+  T *sF = sB + CB * CS;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
   const long long cell0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB;
   const long long cell = traverse_cell(cell0 + c, a.nx, a.ny, a.nz, a.order);
   const bool act = c < CB && cell < a.ncells;
-  T *A = sA + (act ? c : 0) * N3;
-  T *B = sB + (act ? c : 0) * N3;
+  T *A = sA + (act ? c : 0) * CS;
+  T *B = sB + (act ? c : 0) * CS;
   T *F = sF + (KIND != 2 && act ? c * 6 * N2 : 0);
   int nbr[6] = {0, 0, 0, 0, 0, 0}; // periodic neighbour cells
   if (KIND != 2 && act) periodic_neighbors((int)cell, a.nx, a.ny, a.nz, nbr);
@@ -183,40 +185,40 @@ advect_kernel(const AdvArgs<N, T> a)
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
       eo_apply<N, 1>(tb.Ib, u, v);
-      lds_write_line<N, N, 2>(A, l, v);
+      cl_write<N, 2>(A, l, v);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 0>(A, l, u);
+      cl_read<N, 0>(A, l, u);
       eo_apply<N, 1>(tb.Ib, u, v);
-      lds_write_line<N, N, 0>(A, l, v);
+      cl_write<N, 0>(A, l, v);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 1>(A, l, u);
+      cl_read<N, 1>(A, l, u);
       eo_apply<N, 1>(tb.Ib, u, v);
 #pragma unroll
       for (int i = 0; i < N; ++i) v[i] /= a.jxw * line_weight<N, 1>(tb.w, l, i);
       eo_apply<N, 1>(tb.If, v, u);
-      lds_write_line<N, N, 1>(A, l, u);
+      cl_write<N, 1>(A, l, u);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 0>(A, l, u);
+      cl_read<N, 0>(A, l, u);
       eo_apply<N, 1>(tb.If, u, v);
-      lds_write_line<N, N, 0>(A, l, v);
+      cl_write<N, 0>(A, l, v);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 2>(A, l, u);
+      cl_read<N, 2>(A, l, u);
       eo_apply<N, 1>(tb.If, u, v);
 #pragma unroll
       for (int k = 0; k < N; ++k) a.dst[cell * N3 + k * N2 + l] = v[k];
@@ -231,11 +233,11 @@ advect_kernel(const AdvArgs<N, T> a)
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 2>(A, l, v);
+      cl_write<N, 2>(A, l, v);
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.aux[cell * N3 + k * N2 + l];
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 2>(B, l, v);
+      cl_write<N, 2>(B, l, v);
     }
     if (act) {
       T v[6];
@@ -253,24 +255,24 @@ advect_kernel(const AdvArgs<N, T> a)
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 0>(A, l, u);
+      cl_read<N, 0>(A, l, u);
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 0>(A, l, v);
-      lds_read_line<N, N, 0>(B, l, u);
+      cl_write<N, 0>(A, l, v);
+      cl_read<N, 0>(B, l, u);
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 0>(B, l, v);
+      cl_write<N, 0>(B, l, v);
       face_sweep_t<N, T>(tb.Sf, F, 6, l, true);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 1>(A, l, u);
+      cl_read<N, 1>(A, l, u);
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 1>(A, l, v);
-      lds_read_line<N, N, 1>(B, l, u);
+      cl_write<N, 1>(A, l, v);
+      cl_read<N, 1>(B, l, u);
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 1>(B, l, v);
+      cl_write<N, 1>(B, l, v);
       face_sweep_t<N, T>(tb.Sf, F, 6, l, false);
     }
     __syncthreads();
@@ -279,8 +281,8 @@ advect_kernel(const AdvArgs<N, T> a)
       auto do_dir = [&](auto dirc) {
         constexpr int DIR = decltype(dirc)::value;
         T uq[N], mq[N], g[N];
-        lds_read_line<N, N, DIR>(A, l, uq);
-        lds_read_line<N, N, DIR>(B, l, mq);
+        cl_read<N, DIR>(A, l, uq);
+        cl_read<N, DIR>(B, l, mq);
         eo_apply<N, -1>(tb.Df, mq, g);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -314,25 +316,25 @@ advect_kernel(const AdvArgs<N, T> a)
 #pragma unroll
       for (int k = 0; k < N; ++k) u[k] = a.src[cell * N3 + k * N2 + l];
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 2>(A, l, v);
+      cl_write<N, 2>(A, l, v);
     }
     if (act) load_face_layers<N, T>(a.src, F, nbr, l);
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 0>(A, l, u);
+      cl_read<N, 0>(A, l, u);
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 0>(A, l, v);
+      cl_write<N, 0>(A, l, v);
       face_sweep_t<N, T>(tb.Sf, F, 6, l, true);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 1>(A, l, u);
+      cl_read<N, 1>(A, l, u);
       eo_apply<N, 1>(tb.Sf, u, v);
-      lds_write_line<N, N, 1>(A, l, v);
+      cl_write<N, 1>(A, l, v);
       face_sweep_t<N, T>(tb.Sf, F, 6, l, false);
     }
     __syncthreads();
@@ -340,24 +342,24 @@ advect_kernel(const AdvArgs<N, T> a)
       const auto &tb = ctab(a.dt);
       T r[N];
       adv_dir<N, 0, T, VAR, ADJ>(a, A, F, l, cell, nbr[0], r);
-      lds_write_line<N, N, 0>(B, l, r);
+      cl_write<N, 0>(B, l, r);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T r[N], b[N];
       adv_dir<N, 1, T, VAR, ADJ>(a, A, F, l, cell, nbr[2], r);
-      lds_read_line<N, N, 1>(B, l, b);
+      cl_read<N, 1>(B, l, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) r[i] += b[i];
-      lds_write_line<N, N, 1>(B, l, r);
+      cl_write<N, 1>(B, l, r);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T r[N], b[N], o[N];
       adv_dir<N, 2, T, VAR, ADJ>(a, A, F, l, cell, nbr[4], r);
-      lds_read_line<N, N, 2>(B, l, b);
+      cl_read<N, 2>(B, l, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) r[i] += b[i];
       if constexpr (!ADJ) {
@@ -368,22 +370,22 @@ advect_kernel(const AdvArgs<N, T> a)
       } else {
         eo_apply<N, 1>(tb.Sb, r, o);
       }
-      lds_write_line<N, N, 2>(A, l, o);
+      cl_write<N, 2>(A, l, o);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], v[N];
-      lds_read_line<N, N, 1>(A, l, u);
+      cl_read<N, 1>(A, l, u);
       if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, v);
       else eo_apply<N, 1>(tb.Sb, u, v);
-      lds_write_line<N, N, 1>(A, l, v);
+      cl_write<N, 1>(A, l, v);
     }
     __syncthreads();
     if (act) {
       const auto &tb = ctab(a.dt);
       T u[N], o[N];
-      lds_read_line<N, N, 0>(A, l, u);
+      cl_read<N, 0>(A, l, u);
       if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, o);
       else eo_apply<N, 1>(tb.Sb, u, o);
       const long long g0 = cell * N3 + l * N;
@@ -403,6 +405,12 @@ advect_kernel(const AdvArgs<N, T> a)
   }
 }
 
+
+template <int N, typename T>
+static size_t advect_lds_per_cell_n(bool minv)
+{
+  return size_t(2 * CellLds<N, T>::CS + (minv ? 0 : 6 * N * N)) * sizeof(T);
+}
 
 template <int N, typename T>
 static hipError_t launch_advect_n(const AdvectLaunch &L, hipStream_t s)
@@ -434,7 +442,7 @@ static hipError_t launch_advect_n(const AdvectLaunch &L, hipStream_t s)
   if (CB * N * N > kAdvMaxThreads) return hipErrorInvalidConfiguration;
   a.cb = CB;
   a.order = L.order;
-  const size_t lds = size_t(CB) * (2 * N * N * N + (L.kind == 2 ? 0 : 6 * N * N)) * sizeof(T);
+  const size_t lds = size_t(CB) * advect_lds_per_cell_n<N, T>(L.kind == 2);
   const dim3 grid((unsigned)((L.ncells + CB - 1) / CB)), block(((CB * N * N + 63) / 64) * 64);
   const bool var = L.cell_vel != nullptr;
   decltype(&advect_kernel<N, T, 0, false, false>) kern = nullptr;
@@ -463,6 +471,17 @@ static hipError_t dispatch_advect(const AdvectLaunch &L, hipStream_t s)
     SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
 #undef SFK_CASE
   default: return hipErrorInvalidValue;
+  }
+}
+
+size_t advect_lds_per_cell(int n, bool fp32)
+{
+  switch (n) {
+#define SFK_CASE(NN) \
+  case NN: return fp32 ? advect_lds_per_cell_n<NN, float>(false) : advect_lds_per_cell_n<NN, double>(false);
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return size_t(2 * n * n * n + 6 * n * n) * (fp32 ? 4 : 8);
   }
 }
 

@@ -187,6 +187,47 @@ __device__ __forceinline__ void lds_write_line(T *cell, int l, const T (&v)[N])
   for (int i = 0; i < N; ++i) cell[b + i * line_stride<N, NP, DIR>()] = v[i];
 }
 
+// Padded LDS block of a cell (x fastest, rows of N, planes PP values apart,
+// cells CS apart).  LDS serves 8-byte accesses in passes of 16 lanes and
+// 16-byte accesses in passes of 8 lanes over 32 four-byte banks (measured on
+// MI355X with the bank-conflict counter).  For N = 4 in FP64 an unpadded
+// block (PP = 16) puts the 16 y lines of a cell on 4 bank pairs (4-way
+// conflicts) and the x lines' 16-byte accesses 2-way; PP = 18 spreads the x
+// lines over all 16 bank pairs and leaves the y lines 2-way.  Other N and
+// FP32 keep the unpadded block.
+template <int N, typename T>
+struct CellLds {
+  static constexpr bool pad = N == 4 && sizeof(T) == 8;
+  static constexpr int PP = pad ? 18 : N * N;
+  static constexpr int CS = pad ? 3 * 18 + 16 : N * N * N;
+};
+// LDS offset of dof d (x fastest) inside a padded cell block
+template <int N, typename T>
+__device__ __forceinline__ int cl_off(int d)
+{
+  return (d / (N * N)) * CellLds<N, T>::PP + d % (N * N);
+}
+// line l of direction DIR of a padded cell block (x line l = k N + j, y line
+// l = k N + i, z line l = j N + i)
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void cl_read(const T *cell, int l, T (&v)[N])
+{
+  constexpr int PP = CellLds<N, T>::PP;
+  const int b = DIR == 0 ? (l / N) * PP + (l % N) * N : (DIR == 1 ? (l / N) * PP + l % N : l);
+  constexpr int st = DIR == 0 ? 1 : (DIR == 1 ? N : PP);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = cell[b + i * st];
+}
+template <int N, int DIR, typename T>
+__device__ __forceinline__ void cl_write(T *cell, int l, const T (&v)[N])
+{
+  constexpr int PP = CellLds<N, T>::PP;
+  const int b = DIR == 0 ? (l / N) * PP + (l % N) * N : (DIR == 1 ? (l / N) * PP + l % N : l);
+  constexpr int st = DIR == 0 ? 1 : (DIR == 1 ? N : PP);
+#pragma unroll
+  for (int i = 0; i < N; ++i) cell[b + i * st] = v[i];
+}
+
 // quadrature weight of point i on line l of direction DIR (w: the device
 // table; the per-lane entries are vector loads)
 template <int N, int DIR, typename P>

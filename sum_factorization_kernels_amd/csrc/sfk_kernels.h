@@ -83,6 +83,8 @@ struct AdvectLaunch {
   const void *dtab = nullptr;
 };
 hipError_t launch_advect(const AdvectLaunch &L, hipStream_t stream);
+// LDS bytes per cell of launch_advect's operator kernels (n points per direction)
+size_t advect_lds_per_cell(int n, bool fp32);
 
 // DG Laplacian with precomputed 1D matrices (PrecomputedDGLaplacian)
 struct PrecompLaunch {

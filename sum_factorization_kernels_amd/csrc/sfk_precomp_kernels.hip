@@ -51,52 +51,11 @@ constexpr int pc_cells_per_block()
   return kPcMaxThreads / (N * N) > 0 ? kPcMaxThreads / (N * N) : 1;
 }
 
-// LDS layout of a cell's values: x fastest, rows of N, planes PP values
-// apart, cells CS apart.  LDS serves 8-byte accesses in passes of 16 lanes
-// and 16-byte accesses in passes of 8 lanes over 32 four-byte banks
-// (measured on MI355X).  For N = 4 in FP64 the unpadded cell (PP = 16)
-// puts the 16 y lines of a cell on 4 bank pairs (4-way conflicts) and the x
-// lines' 16-byte accesses 2-way; PP = 18 spreads the x lines over all 16
-// bank pairs and leaves the y lines 2-way (conflict cycles per wave 448 ->
-// 112 in that model).
-template <int N, typename T>
-struct PcLds {
-  static constexpr bool pad = N == 4 && sizeof(T) == 8;
-  static constexpr int PP = pad ? 18 : N * N;
-  static constexpr int CS = pad ? 3 * 18 + 16 : N * N * N;
-};
-
-// LDS offset of the dof d (x fastest) of a cell
-template <int N, typename T>
-__device__ __forceinline__ int pc_off(int d)
-{
-  return (d / (N * N)) * PcLds<N, T>::PP + d % (N * N);
-}
-
-template <int N, typename T, int DIR>
-__device__ __forceinline__ void pc_read(const T *cell, int l, T (&v)[N])
-{
-  constexpr int PP = PcLds<N, T>::PP;
-  const int b = DIR == 0 ? (l / N) * PP + (l % N) * N : (DIR == 1 ? (l / N) * PP + l % N : l);
-  constexpr int st = DIR == 0 ? 1 : (DIR == 1 ? N : PP);
-#pragma unroll
-  for (int i = 0; i < N; ++i) v[i] = cell[b + i * st];
-}
-template <int N, typename T, int DIR>
-__device__ __forceinline__ void pc_write(T *cell, int l, const T (&v)[N])
-{
-  constexpr int PP = PcLds<N, T>::PP;
-  const int b = DIR == 0 ? (l / N) * PP + (l % N) * N : (DIR == 1 ? (l / N) * PP + l % N : l);
-  constexpr int st = DIR == 0 ? 1 : (DIR == 1 ? N : PP);
-#pragma unroll
-  for (int i = 0; i < N; ++i) cell[b + i * st] = v[i];
-}
-
 template <int N, typename T>
 constexpr int pc_waves()
 {
   constexpr int cb = pc_cells_per_block<N>();
-  return lds_waves_per_simd(3 * cb * PcLds<N, T>::CS * (int)sizeof(T), cb * N * N);
+  return lds_waves_per_simd(3 * cb * CellLds<N, T>::CS * (int)sizeof(T), cb * N * N);
 }
 
 template <int N, typename T>
@@ -148,7 +107,7 @@ precomp_kernel(PcArgs<N, T> a)
 {
   constexpr int N2 = N * N, N3 = N2 * N;
   const int cb = a.cb;
-  constexpr int CS = PcLds<N, T>::CS;
+  constexpr int CS = CellLds<N, T>::CS;
   T *U = dyn_lds<T>();
   T *P = U + cb * CS, *Q = P + cb * CS;
   const long long c0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * cb;
@@ -181,7 +140,7 @@ precomp_kernel(PcArgs<N, T> a)
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int i = t + k * (int)blockDim.x;
-    if (i < nval) U[(i / N3) * CS + pc_off<N, T>(i % N3)] = keep[k];
+    if (i < nval) U[(i / N3) * CS + cl_off<N, T>(i % N3)] = keep[k];
   }
   __syncthreads();
 
@@ -190,19 +149,19 @@ precomp_kernel(PcArgs<N, T> a)
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    pc_read<N, T, 0>(Uc, l, v);
+    cl_read<N, 0>(Uc, l, v);
     eo_apply<N, 1>(pt.L, v, o);
     if constexpr (COUPLE) pc_couple(o, nb[0], pt.v1, pt.v2);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] *= pt.f[0];
-    pc_write<N, T, 0>(Pc, l, o);
-    pc_read<N, T, 1>(Uc, l, v);
+    cl_write<N, 0>(Pc, l, o);
+    cl_read<N, 1>(Uc, l, v);
     eo_apply<N, 1>(pt.L, v, o);
     if constexpr (COUPLE) pc_couple(o, nb[1], pt.v1, pt.v2);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] *= pt.f[1];
-    pc_write<N, T, 1>(Qc, l, o);
-    pc_read<N, T, 2>(Uc, l, v);
+    cl_write<N, 1>(Qc, l, o);
+    cl_read<N, 2>(Uc, l, v);
     eo_apply<N, 1>(pt.L, v, r);
     if constexpr (COUPLE) pc_couple(r, nb[2], pt.v1, pt.v2);
 #pragma unroll
@@ -212,38 +171,38 @@ precomp_kernel(PcArgs<N, T> a)
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    pc_read<N, T, 2>(Pc, l, v);
+    cl_read<N, 2>(Pc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    pc_write<N, T, 2>(Pc, l, o);
-    pc_read<N, T, 2>(Qc, l, v);
+    cl_write<N, 2>(Pc, l, o);
+    cl_read<N, 2>(Qc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    pc_write<N, T, 2>(Qc, l, o);
-    pc_write<N, T, 2>(Uc, l, r);
+    cl_write<N, 2>(Qc, l, o);
+    cl_write<N, 2>(Uc, l, r);
   }
   __syncthreads();
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    pc_read<N, T, 1>(Pc, l, v);
+    cl_read<N, 1>(Pc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    pc_write<N, T, 1>(Pc, l, o);
-    pc_read<N, T, 1>(Uc, l, v);
+    cl_write<N, 1>(Pc, l, o);
+    cl_read<N, 1>(Uc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    pc_read<N, T, 1>(Qc, l, v);
+    cl_read<N, 1>(Qc, l, v);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] = v[i] + o[i];
-    pc_write<N, T, 1>(Qc, l, o);
+    cl_write<N, 1>(Qc, l, o);
   }
   __syncthreads();
   if (act) {
     const auto &pt = pctab(a.pt);
     T v[N], o[N];
-    pc_read<N, T, 0>(Qc, l, v);
+    cl_read<N, 0>(Qc, l, v);
     eo_apply<N, 1>(pt.M, v, o);
-    pc_read<N, T, 0>(Pc, l, v);
+    cl_read<N, 0>(Pc, l, v);
 #pragma unroll
     for (int i = 0; i < N; ++i) o[i] = o[i] + v[i];
-    pc_write<N, T, 0>(Uc, l, o);
+    cl_write<N, 0>(Uc, l, o);
   }
   __syncthreads();
   const long long g0 = voff + c0 * N3;
@@ -251,7 +210,7 @@ precomp_kernel(PcArgs<N, T> a)
   for (int k = 0; k < N; ++k) {
     const int i = t + k * (int)blockDim.x;
     if (i < nval) {
-      const T au = U[(i / N3) * CS + pc_off<N, T>(i % N3)];
+      const T au = U[(i / N3) * CS + cl_off<N, T>(i % N3)];
       if constexpr (CHEBY) {
         // r = A u - rhs; tm = c_tm tm + c_np diag r; dst = u - tm (tm not
         // read when c_tm == 0)
@@ -305,7 +264,7 @@ hipError_t launch_precomp_n(const PrecompLaunch &L, hipStream_t s)
   if (L.cheby && L.nrhs != 1) return hipErrorInvalidValue;
   a.cb = CB;
   a.order = L.order;
-  const size_t lds = size_t(3) * CB * PcLds<N, T>::CS * sizeof(T);
+  const size_t lds = size_t(3) * CB * CellLds<N, T>::CS * sizeof(T);
   const dim3 grid((unsigned)((L.ncells + CB - 1) / CB), (unsigned)L.nrhs), block(((CB * N * N + 63) / 64) * 64);
   decltype(&precomp_kernel<N, T, true, false>) kern;
   if (L.couple) kern = L.cheby ? precomp_kernel<N, T, true, true> : precomp_kernel<N, T, true, false>;
@@ -319,7 +278,7 @@ hipError_t launch_precomp_n(const PrecompLaunch &L, hipStream_t s)
 template <int N, typename T>
 size_t precomp_lds_per_cell_n()
 {
-  return size_t(3) * PcLds<N, T>::CS * sizeof(T);
+  return size_t(3) * CellLds<N, T>::CS * sizeof(T);
 }
 
 template <int N, typename T>

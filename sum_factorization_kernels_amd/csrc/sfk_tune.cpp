@@ -44,7 +44,7 @@ size_t lds_per_cell(const sfk_op *op)
   switch (op->d.kind) {
   case SFK_DG_SIPG:
     return (op->d.geometry == SFK_GEOM_PER_Q ? 4 * n3 + 12 * n2 : 2 * n3 + 12 * n2) * es;
-  case SFK_DG_ADVECT: return (2 * n3 + 6 * n2) * es;
+  case SFK_DG_ADVECT: return advect_lds_per_cell((int)n, es == 4);
   case SFK_DG_PRECOMP: return precomp_lds_per_cell((int)n, es == 4);
   default: return 2 * n3 * es;
   }
