@@ -193,13 +193,15 @@ __device__ __forceinline__ void lds_write_line(T *cell, int l, const T (&v)[N])
 // MI355X with the bank-conflict counter).  For N = 4 in FP64 an unpadded
 // block (PP = 16) puts the 16 y lines of a cell on 4 bank pairs (4-way
 // conflicts) and the x lines' 16-byte accesses 2-way; PP = 18 spreads the x
-// lines over all 16 bank pairs and leaves the y lines 2-way.  Other N and
-// FP32 keep the unpadded block.
+// lines over all 16 bank pairs and leaves the y lines 2-way.  For N = 4 in
+// FP32, PP = 20 with cells 80 values apart measured fastest of the layouts
+// tried (precomputed DG p = 3: 29 -> 25 us; planes 36 apart: 28 us).  Other
+// N keep the unpadded block.
 template <int N, typename T>
 struct CellLds {
-  static constexpr bool pad = N == 4 && sizeof(T) == 8;
-  static constexpr int PP = pad ? 18 : N * N;
-  static constexpr int CS = pad ? 3 * 18 + 16 : N * N * N;
+  static constexpr bool pad = N == 4;
+  static constexpr int PP = pad ? (sizeof(T) == 8 ? 18 : 20) : N * N;
+  static constexpr int CS = pad ? (sizeof(T) == 8 ? 3 * 18 + 16 : 80) : N * N * N;
 };
 // LDS offset of dof d (x fastest) inside a padded cell block
 template <int N, typename T>
