@@ -299,6 +299,7 @@ def main():
         o4 = sfk.Operator(sfk.DG_ADVECT, 3, (48, 48, 48), inv_jac=(48, 48, 48), flags=sfk.FLAG_VARIABLE_VELOCITY)
         cv, fv = advection_velocity(3, (48, 48, 48))
         o4.set_velocity(cv, fv)
+        sched4, _ = o4.tune(repeats=5)
         u4 = torch.randn(o4.vector_size, dtype=torch.float64, device="cuda",
                          generator=torch.Generator("cuda").manual_seed(4))
         v4 = torch.empty_like(u4)
@@ -307,7 +308,7 @@ def main():
         t4g = timed(lambda: o4.grad_velocity(u4, v4), 20, 3)
         extra["advect_p3_48cube"] = {"forward_gdofs": round(o4.vector_size / t4 / 1e9, 3),
                                      "adjoint_gdofs": round(o4.vector_size / t4a / 1e9, 3),
-                                     "velocity_grad_ms": round(t4g * 1e3, 4)}
+                                     "velocity_grad_ms": round(t4g * 1e3, 4), "schedule": list(sched4)}
         log(f"advection p3 48^3: fwd {o4.vector_size / t4 / 1e9:.2f} GDoF/s, adj {o4.vector_size / t4a / 1e9:.2f}")
         del o4, u4, v4
         # c5: variable-coefficient Q8 24^3, 8 RHS in one launch, FP32
