@@ -125,8 +125,8 @@ def main():
                     help="plain copy kernel of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the operators shard by cells, but no multi-GPU "
-              "path is built); use --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU only (the operators would shard by cells, but the library has "
+              "no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
