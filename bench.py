@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="headline matvec only (profiling)")
+    ap.add_argument("--only-precomputed", action="store_true",
+                    help="precomputed-matrix DG p3 56x50x55 FP64 matvec only, cells per workgroup 4 (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="plain copy kernel of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -142,6 +144,15 @@ def main():
             torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 5}))
+        return
+
+    if args.only_precomputed:
+        op = sfk.Operator(sfk.DG_PRECOMP, 3, (56, 50, 55))
+        op.set_schedule(4, 1, -1)
+        x = torch.randn(op.vector_size, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        t = timed(lambda: op.apply(x, y), args.steps, args.warmup)
+        log(f"precomputed DG p3 56x50x55 FP64: {t * 1e6:.1f} us, {op.vector_size / t / 1e9:.2f} GDoF/s")
         return
 
     # ------------------------------------------------------------ headline
