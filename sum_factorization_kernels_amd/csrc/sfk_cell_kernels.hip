@@ -239,13 +239,22 @@ cell_kernel(const CellArgs<N, T> a)
       cell0 = ((long long)ccz * a.ny + ccy) * a.nx + cx0;
       act = c < CB && ccx < cx_end;
     } else {
-      const int ci = (int)(cell0 + c);
-      ccx = ci % a.nx;
-      ccy = (ci / a.nx) % a.ny;
-      ccz = ci / a.nx / a.ny;
+      // cell0's coordinates are uniform; cell0 + c is c steps further in x
+      // unless that crosses the end of the row
+      CellXYZ q = traverse_xyz((unsigned)cell0, a.nx, a.ny, a.nz, 0);
+      if (q.x + c < (unsigned)a.nx) q.x += c;
+      else q = traverse_xyz((unsigned)(cell0 + c), a.nx, a.ny, a.nz, 0);
+      ccx = (int)q.x;
+      ccy = (int)q.y;
+      ccz = (int)q.z;
     }
   }
-  const long long cell = MODE == Mode::DG ? traverse_cell(cell0 + c, a.nx, a.ny, a.nz, a.order) : cell0 + c;
+  // DG: the cell at this traversal position and its periodic face neighbours
+  int nbr[6] = {0, 0, 0, 0, 0, 0};
+  long long cell = cell0 + c;
+  if constexpr (MODE == Mode::DG) {
+    if (act) cell = locate_cell((unsigned)cell0, (unsigned)c, a.nx, a.ny, a.nz, a.order, nbr);
+  }
   T *A = sA + (act ? c : 0) * N3;
   T *B = sB + (act ? c : 0) * N3;
   T *F = sF + (FACES && act ? c * 12 * N2 : 0);
@@ -285,8 +294,6 @@ cell_kernel(const CellArgs<N, T> a)
       // thread l of a cell loads face point l of all six faces (twelve
       // independent loads, issued together)
       if (act) {
-        int nbr[6];
-        periodic_neighbors((int)cell, a.nx, a.ny, a.nz, nbr);
         T uf[6], ui[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {

@@ -295,6 +295,55 @@ __device__ __forceinline__ long long traverse_cell(long long q, int nx, int ny, 
   return (long long)(((z0 + zz) * (unsigned)ny + y) * unx + x);
 }
 
+// Coordinates (x, y, z) of the cell at traversal position q (traverse_cell's
+// order; cell = (z ny + y) nx + x).
+struct CellXYZ {
+  unsigned x, y, z;
+};
+__device__ __forceinline__ CellXYZ traverse_xyz(unsigned q, unsigned nx, unsigned ny, unsigned nz, int order)
+{
+  CellXYZ p;
+  if (order != 2) {
+    const unsigned r = q / nx;
+    p.x = q - r * nx;
+    p.y = r % ny;
+    p.z = r / ny;
+  } else {
+    const unsigned layer = nx * ny;
+    const unsigned z0 = q / layer / kSlab * kSlab;
+    const unsigned zs = nz - z0 < (unsigned)kSlab ? nz - z0 : (unsigned)kSlab;
+    const unsigned r = q - z0 * layer, r2 = r / nx;
+    p.x = r - r2 * nx;
+    p.y = r2 / zs;
+    p.z = z0 + (r2 - p.y * zs);
+  }
+  return p;
+}
+
+// Cell c of a workgroup whose first position q0 is the same for all its
+// threads: q0's coordinates are uniform (scalar unit), and position q0 + c
+// of both traversals is q0's cell moved c steps in x unless that crosses
+// the end of the x row (then the divisions are done per thread).  Returns
+// the cell index; its periodic face neighbours go to nbr (as
+// periodic_neighbors).
+__device__ __forceinline__ long long locate_cell(unsigned q0, unsigned c, int nx, int ny, int nz, int order,
+                                                 int (&nbr)[6])
+{
+  const unsigned unx = (unsigned)nx, uny = (unsigned)ny, unz = (unsigned)nz;
+  CellXYZ p = traverse_xyz(q0, unx, uny, unz, order);
+  if (p.x + c < unx) p.x += c;
+  else p = traverse_xyz(q0 + c, unx, uny, unz, order);
+  const int sy = nx, sz = nx * ny;
+  const int ci = ((int)p.z * ny + (int)p.y) * nx + (int)p.x;
+  nbr[0] = ci + (p.x == 0 ? nx - 1 : -1);
+  nbr[1] = ci + (p.x == unx - 1 ? 1 - nx : 1);
+  nbr[2] = ci + (p.y == 0 ? (ny - 1) * sy : -sy);
+  nbr[3] = ci + (p.y == uny - 1 ? -(ny - 1) * sy : sy);
+  nbr[4] = ci + (p.z == 0 ? (nz - 1) * sz : -sz);
+  nbr[5] = ci + (p.z == unz - 1 ? -(nz - 1) * sz : sz);
+  return ci;
+}
+
 // Waves per SIMD that a kernel's LDS allows at its default cells per
 // workgroup (gfx950: 160 KiB of LDS per CU, allocated to workgroups in
 // 1280-byte granules -- measured on MI355X with fixed-duration workgroups:
