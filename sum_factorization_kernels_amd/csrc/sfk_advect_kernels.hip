@@ -169,13 +169,13 @@ advect_kernel(const AdvArgs<N, T> a)
   T *sF = sB + CB * CS;
   const int t = threadIdx.x, c = t / N2, l = t % N2;
   const long long cell0 = (long long)logical_block(blockIdx.x, gridDim.x, a.order) * CB;
-  const long long cell = traverse_cell(cell0 + c, a.nx, a.ny, a.nz, a.order);
-  const bool act = c < CB && cell < a.ncells;
+  const bool act = c < CB && cell0 + c < a.ncells;
+  // the cell at this traversal position and its periodic neighbour cells
+  int nbr[6] = {0, 0, 0, 0, 0, 0};
+  const long long cell = act ? locate_cell((unsigned)cell0, (unsigned)c, a.nx, a.ny, a.nz, a.order, nbr) : 0;
   T *A = sA + (act ? c : 0) * CS;
   T *B = sB + (act ? c : 0) * CS;
   T *F = sF + (KIND != 2 && act ? c * 6 * N2 : 0);
-  int nbr[6] = {0, 0, 0, 0, 0, 0}; // periodic neighbour cells
-  if (KIND != 2 && act) periodic_neighbors((int)cell, a.nx, a.ny, a.nz, nbr);
 
   if constexpr (KIND == 2) {
     // M^{-1} x = Sinv^T W^{-1} Sinv x
