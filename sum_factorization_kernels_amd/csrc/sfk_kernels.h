@@ -151,6 +151,8 @@ hipError_t vec_cg_update(long long n, double alpha, const void *p, const void *q
 hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, const void *q, double *result_dev,
                           double *partials, int n_partials, hipStream_t s);
 hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, void *y, hipStream_t s);
+// p = (double) zs + b p (zs FP32, p FP64)
+hipError_t vec_axpby_widen(long long n, const void *zs, double b, void *p, hipStream_t s);
 hipError_t vec_cheby_update(bool fp32, long long n, const void *au, const void *rhs, const void *sol_old,
                             const void *diag, void *sol_new, void *tm, double c_np, double c_tm,
                             hipStream_t s);

@@ -208,20 +208,20 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   const size_t n8 = (size_t)n * 8, ns = (size_t)n * (f32 ? 4 : 8);
   // device work space: FP64 vectors, and smoother-precision vectors
   DevBuf red, d64, inv64, isq, r, w, lv0, lv1, lw, lt, e64;
-  DevBuf rs, zs, qs, zero, tms, inv_s;
+  DevBuf rs, zs, qs, tms, inv_s;
   SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (2 * kPartials + 2)));
   S.partials = (double *)red.p;
   S.dres = S.partials + 2 * kPartials; // after the partials of two sums: two result slots
   for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
     SOLVER_HIP(hipMalloc(&bb->p, n8));
-  for (DevBuf *bb : {&rs, &zs, &qs, &zero, &tms, &inv_s})
+  for (DevBuf *bb : {&rs, &zs, &qs, &tms, &inv_s})
     SOLVER_HIP(hipMalloc(&bb->p, ns));
   const hipStream_t s = S.s;
   // every work vector starts at 0 (a fresh allocation may hold NaN bit
   // patterns, and 0 * NaN would poison the first Chebyshev step)
   for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
     SOLVER_HIP(vec_fill(false, bb->p, n, 0., s));
-  for (DevBuf *bb : {&rs, &zs, &qs, &zero, &tms, &inv_s})
+  for (DevBuf *bb : {&rs, &zs, &qs, &tms, &inv_s})
     SOLVER_HIP(vec_fill(f32, bb->p, n, 0., s));
   // point-Jacobi diagonal (FP64), inverse in both precisions
   SOLVER_ST(sfk_compute_diagonal(op, d64.p, stream));
@@ -268,8 +268,9 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   // buffers, the result ends in *z_io (pointers swapped as needed)
   auto cheby = [&](const void *rr, void **z_io, void **spare) -> sfk_status {
     void *z = *z_io;
-    // step 0: tm = (1/theta) D^{-1}(0 - rr), z = 0 - tm
-    SOLVER_HIP(vec_cheby_update(f32, n, zero.p, rr, zero.p, inv_sm, z, tms.p, 1. / theta, 0., s));
+    // step 0: tm = (1/theta) D^{-1}(0 - rr), z = 0 - tm (null: the zero
+    // vector, not read)
+    SOLVER_HIP(vec_cheby_update(f32, n, nullptr, rr, nullptr, inv_sm, z, tms.p, 1. / theta, 0., s));
     double rho_old = 1. / sigma1;
     for (int j = 1; j < cheby_degree; ++j) {
       const double rho = 1. / (2. * sigma1 - rho_old);
@@ -300,21 +301,24 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   double res = S.st.initial_residual;
   int iters = 0;
 
-  // preconditioner z = P(r) (FP64 out): the Chebyshev polynomial on the
-  // smoother operator, in the smoother's precision.  FP32: rs already holds
-  // (float) r and the result is widened into e64 together with the inner
-  // products z.r and z.q (zr_zq != nullptr).  *z64 receives the FP64 buffer
-  // that holds z.
-  auto precond = [&](const void *r64, const void *qv, void **z64, double *zr_zq) -> sfk_status {
+  // preconditioner z = P(r): the Chebyshev polynomial on the smoother
+  // operator, in the smoother's precision.  FP32: rs already holds (float) r.
+  // With zr_zq (the inner products z.r and z.q wanted), an FP32 z stays FP32
+  // (*z32; widened where it is used), otherwise it is widened into e64.
+  // *z64 receives the FP64 buffer that holds z (null when only *z32 does).
+  auto precond = [&](const void *r64, const void *qv, void **z64, double *zr_zq,
+                     const void **z32 = nullptr) -> sfk_status {
     if (f32) {
       void *z = zs.p, *spare = qs.p;
       SOLVER_ST(cheby(rs.p, &z, &spare));
       if (zr_zq) {
-        SOLVER_HIP(vec_widen_dot2(n, z, e64.p, r64, qv, S.dres, S.partials, kPartials, s));
+        SOLVER_HIP(vec_widen_dot2(n, z, nullptr, r64, qv, S.dres, S.partials, kPartials, s));
         SOLVER_ST(S.results(zr_zq, 2));
-      } else {
-        SOLVER_HIP(vec_copy_convert(false, z, e64.p, n, s));
+        *z64 = nullptr;
+        *z32 = z;
+        return SFK_OK;
       }
+      SOLVER_HIP(vec_copy_convert(false, z, e64.p, n, s));
       *z64 = e64.p;
     } else {
       void *z = e64.p, *spare = zs.p;
@@ -369,10 +373,12 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         continue;
       }
       double zrq[2];
-      SOLVER_ST(precond(r.p, qv, &z64, zrq));
+      const void *z32 = nullptr;
+      SOLVER_ST(precond(r.p, qv, &z64, zrq, &z32));
       const double beta_cg = -alpha * zrq[1] / rz;
       rz = zrq[0];
-      SOLVER_HIP(vec_axpby(false, n, 1., z64, beta_cg, pv, s));
+      if (z64) SOLVER_HIP(vec_axpby(false, n, 1., z64, beta_cg, pv, s));
+      else SOLVER_HIP(vec_axpby_widen(n, z32, beta_cg, pv, s));
     }
   }
   // final FP64 residual

@@ -114,7 +114,8 @@ __global__ void __launch_bounds__(kThreads) cg_update_kernel(long long n, double
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-// z = (double) zs with per-block partial sums of z.r and z.q
+// z = (double) zs (not written when z is null) with per-block partial sums
+// of z.r and z.q
 __global__ void __launch_bounds__(kThreads) widen_dot2_kernel(long long n, const float *zs, double *z,
                                                               const double *r, const double *q, double *partials)
 {
@@ -122,7 +123,7 @@ __global__ void __launch_bounds__(kThreads) widen_dot2_kernel(long long n, const
   double a = 0., b = 0.;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double zi = double(zs[i]);
-    z[i] = zi;
+    if (z) z[i] = zi;
     a += zi * r[i];
     b += zi * q[i];
   }
@@ -149,16 +150,25 @@ __global__ void axpby_kernel(long long n, T a, const T *x, T b, T *y)
     y[i] = b != T(0) ? a * x[i] + b * y[i] : a * x[i]; // b == 0: y is write-only
 }
 
+// p = (double) zs + b p
+__global__ void axpby_widen_kernel(long long n, const float *zs, double b, double *p)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = double(zs[i]) + b * p[i];
+}
+
 template <typename T>
 __global__ void cheby_kernel(long long n, const T *au, const T *rhs, const T *sol_old, const T *diag,
                              T *sol_new, T *tm, T c_np, T c_tm)
 {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const T res = au[i] - rhs[i];
+    // a null au or sol_old stands for the zero vector (the first step of a
+    // recurrence started from zero): not read
+    const T res = (au ? au[i] : T(0)) - rhs[i];
     // c_tm == 0 starts a recurrence: the previous tm is not read
     const T t = (c_tm != T(0) ? c_tm * tm[i] : T(0)) + c_np * diag[i] * res;
     tm[i] = t;
-    sol_new[i] = sol_old[i] - t;
+    sol_new[i] = (sol_old ? sol_old[i] : T(0)) - t;
   }
 }
 
@@ -271,6 +281,12 @@ hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, c
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials + g, (int)g,
                      result_dev + 1);
+  return hipGetLastError();
+}
+
+hipError_t vec_axpby_widen(long long n, const void *zs, double b, void *p, hipStream_t s)
+{
+  hipLaunchKernelGGL(axpby_widen_kernel, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, (const float *)zs, b, (double *)p);
   return hipGetLastError();
 }
 
