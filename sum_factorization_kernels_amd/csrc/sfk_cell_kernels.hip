@@ -452,8 +452,9 @@ cell_kernel(const CellArgs<N, T> a)
         }
       } else {
         const long long g0 = cell * N3 + l * N;
+        // written once, read by nobody in this launch: non-temporal stores
 #pragma unroll
-        for (int i = 0; i < N; ++i) dst[g0 + i] = o[i];
+        for (int i = 0; i < N; ++i) __builtin_nontemporal_store(o[i], dst + g0 + i);
       }
     }
   }

@@ -221,7 +221,7 @@ precomp_kernel(PcArgs<N, T> a)
         a.tm[gi] = tmv;
         a.dst[gi] = keep[k] - tmv;
       } else {
-        a.dst[g0 + i] = au;
+        __builtin_nontemporal_store(au, a.dst + g0 + i); // written once
       }
     }
   }

@@ -425,8 +425,11 @@ sipg_general_kernel(const GenArgs<N, T> a)
           a.dst[g] = csrc[i] - tmv;
         }
       } else {
+        // written once, read by nobody in this launch: non-temporal stores
+        // keep L2 for the neighbour layers and face metrics other
+        // workgroups re-read (measured 3-4 % faster on MI355X)
 #pragma unroll
-        for (int i = 0; i < N; ++i) a.dst[g0 + i] = o[i];
+        for (int i = 0; i < N; ++i) __builtin_nontemporal_store(o[i], a.dst + g0 + i);
       }
     }
   }
