@@ -158,6 +158,23 @@ __device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&
     for (int i = 0; i < N; ++i)
       if (i > 0 || c == 0) row[i] = o[i];
   }
+  // per-row terms of the segment's N2 node rows, made once per row instead
+  // of once per node: the index of the row's first node, a Dirichlet-row
+  // bit (4), and the y/z part of the smallest colour of the units covering
+  // the row (rows [ylo,yhi] x [zlo,zhi], each range one or two entries; the
+  // segment part is added per node)
+  __shared__ long long s_rbase[N2];
+  __shared__ int s_rflag[N2];
+  if ((int)threadIdx.x < N2) {
+    const int rw = (int)threadIdx.x, j = rw % N, k = rw / N;
+    const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
+    const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;
+    const int py = ylo == yhi ? (ylo & 1) : 0, pz = zlo == zhi ? (zlo & 1) : 0;
+    const int Y = cy * p + j, Z = cz * p + k;
+    const bool bd = Y == 0 || Z == 0 || Y == a.gy - 1 || Z == a.gz - 1;
+    s_rbase[rw] = ((long long)Z * a.gy + Y) * a.gx + (long long)cx0 * p;
+    s_rflag[rw] = py + 2 * pz + (bd ? 4 : 0);
+  }
   __syncthreads();
   // blockDim.x >= CB (p+1)^2 and N2 (ncs p + 1) <= CB N3: N rounds cover the rows
   const int total = N2 * Lx;
@@ -177,18 +194,13 @@ __device__ __forceinline__ void cont_store_colored(const CellArgs<N, T> &a, T (&
     old[r] = T(0);
     if (idx < total) {
       const int rw = (int)((unsigned(idx) * mlx) >> 20), xi = idx - rw * Lx;
-      const int j = rw % N, k = rw / N;
-      const int ylo = (j == 0 && cy > 0) ? cy - 1 : cy, yhi = (j == p && cy + 1 < a.ny) ? cy + 1 : cy;
-      const int zlo = (k == 0 && cz > 0) ? cz - 1 : cz, zhi = (k == p && cz + 1 < a.nz) ? cz + 1 : cz;
-      // smallest colour of the units covering rows [ylo,yhi] x [zlo,zhi] x
-      // segments [slo,shi] (each range has one or two entries)
-      const int py = ylo == yhi ? (ylo & 1) : 0, pz = zlo == zhi ? (zlo & 1) : 0;
+      const int fl = s_rflag[rw];
       int ps = seg & 1;
       if ((xi == 0 && seg > 0) || (xi == Lx - 1 && seg + 1 < a.nseg)) ps = 0;
-      const int m = py + 2 * pz + 4 * ps;
-      const int X = cx0 * p + xi, Y = cy * p + j, Z = cz * p + k;
-      const bool bd = X == 0 || Y == 0 || Z == 0 || X == a.gx - 1 || Y == a.gy - 1 || Z == a.gz - 1;
-      gi[r] = ((long long)Z * a.gy + Y) * a.gx + X;
+      const int m = (fl & 3) + 4 * ps;
+      const int X = cx0 * p + xi;
+      const bool bd = (fl & 4) || X == 0 || X == a.gx - 1;
+      gi[r] = s_rbase[rw] + xi;
       val[r] = bd ? T(0) : sA[idx];
       mode[r] = m == a.color ? 1 : (bd ? 0 : 2);
     }
