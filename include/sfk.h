@@ -182,7 +182,14 @@ typedef struct sfk_schedule {
                           2 as 1 with the DG kernels walking slabs of 4
                           z-layers x, z, y (neighbour data re-read from L2) */
   int cont_scatter;    /* continuous Laplacian: -1 automatic, 0 zero fill +
-                          atomic adds, 1 colored stores (8 launches) */
+                          atomic adds, 1 colored stores (8 launches), 2 the
+                          tensor-product kernel (constant coefficient only:
+                          assembled 1D mass/stiffness sweeps on the node
+                          grid, no scatter; the automatic choice then) */
+  int tile_shape;      /* cont_scatter 2: tile of cells in x/y per workgroup,
+                          an index < sfk_op_tile_shapes (0 = default) */
+  int z_cells;         /* cont_scatter 2: cells in z per workgroup (0 =
+                          automatic) */
 } sfk_schedule;
 
 /* Current schedule, with defaults resolved (cells_per_block > 0). */
@@ -193,13 +200,18 @@ sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s);
 /* Largest cells_per_block of the operator (workgroup of at most 256 threads,
  * LDS within the device's per-workgroup limit). */
 sfk_status sfk_op_max_cells_per_block(const sfk_op *op, int *max_cells);
+/* Number of tile shapes of the continuous tensor-product kernel and the
+ * shape's cells in x and y (bx, by may be NULL). */
+sfk_status sfk_op_tile_shape(const sfk_op *op, int shape, int *count, int *bx, int *by);
 /* Times sfk_apply for every candidate schedule (`repeats` applies each,
  * after a warm-up apply, on internal scratch vectors; stream-ordered on
  * `stream`), sets the fastest on the handle and returns it in *best with its
  * time per apply in *best_seconds (either may be NULL).  Candidates: every
  * cells_per_block in [1, max] x both block orders x, for the continuous
- * Laplacian, both scatters.  Requires the operator's data (metric,
- * coefficient, velocity) to be set. */
+ * Laplacian, both scatters, and with a constant coefficient every tile shape
+ * x z_cells (powers of two up to the cells in z) of the tensor-product
+ * kernel.  Requires the operator's data (metric, coefficient, velocity) to
+ * be set. */
 sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_seconds, void *stream);
 
 /* ---- solver (continuous Laplacian) ------------------------------------

@@ -50,7 +50,10 @@ class SolveStats(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     _fields_ = [("cells_per_block", ctypes.c_int), ("block_order", ctypes.c_int),
-                ("cont_scatter", ctypes.c_int)]
+                ("cont_scatter", ctypes.c_int), ("tile_shape", ctypes.c_int), ("z_cells", ctypes.c_int)]
+
+    def astuple(self):
+        return (self.cells_per_block, self.block_order, self.cont_scatter, self.tile_shape, self.z_cells)
 
 
 _lib = None
@@ -101,6 +104,7 @@ def lib():
         L.sfk_op_get_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
         L.sfk_op_set_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
         L.sfk_op_max_cells_per_block.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+        L.sfk_op_tile_shape.argtypes = [_vp, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.sfk_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Schedule), _dp, _vp]
         L.sfk_cheby_sweep.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, _vp]
@@ -285,14 +289,20 @@ class Operator:
 
     @property
     def schedule(self):
-        """(cells_per_block, block_order, cont_scatter) of the operator kernel"""
+        """(cells_per_block, block_order, cont_scatter, tile_shape, z_cells) of the operator kernel"""
         sc = Schedule()
         check(lib().sfk_op_get_schedule(self.handle, ctypes.byref(sc)))
-        return (sc.cells_per_block, sc.block_order, sc.cont_scatter)
+        return sc.astuple()
 
-    def set_schedule(self, cells_per_block=0, block_order=1, cont_scatter=-1):
-        sc = Schedule(int(cells_per_block), int(block_order), int(cont_scatter))
+    def set_schedule(self, cells_per_block=0, block_order=1, cont_scatter=-1, tile_shape=0, z_cells=0):
+        sc = Schedule(int(cells_per_block), int(block_order), int(cont_scatter), int(tile_shape), int(z_cells))
         check(lib().sfk_op_set_schedule(self.handle, ctypes.byref(sc)))
+
+    def tile_shape(self, shape):
+        """(count, cells in x, cells in y) of tile shape `shape` of the continuous tensor-product kernel"""
+        c, bx, by = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().sfk_op_tile_shape(self.handle, int(shape), ctypes.byref(c), ctypes.byref(bx), ctypes.byref(by)))
+        return c.value, bx.value, by.value
 
     @property
     def max_cells_per_block(self):
@@ -305,7 +315,7 @@ class Operator:
         sc = Schedule()
         t = ctypes.c_double()
         check(lib().sfk_tune(self.handle, int(repeats), ctypes.byref(sc), ctypes.byref(t), _stream(stream)))
-        return (sc.cells_per_block, sc.block_order, sc.cont_scatter), t.value
+        return sc.astuple(), t.value
 
     def cheby_sweep(self, rhs, x, spare, diag_inv, tmp, degree, lambda_min, lambda_max, stream=None):
         """`degree` fused Chebyshev steps on [lambda_min, lambda_max]; x updated in place"""

@@ -87,13 +87,23 @@ static hipError_t upload_precomp(sfk_op *op, void **dev)
   return upload_precomp_tables(op->d.degree + 1, pc, f, op->d.precision == SFK_FP32, dev);
 }
 
+// the continuous Laplacian with a constant coefficient has the
+// tensor-product kernel as well
+static bool has_tp(const sfk_op *op)
+{
+  return op->d.kind == SFK_CONT_LAPLACE && !(op->d.flags & SFK_FLAG_VARIABLE_COEF);
+}
+
 sfk_status ensure_device_tables(sfk_op *op)
 {
-  if (op->dtab) return SFK_OK;
-  const hipError_t e = op->d.kind == SFK_DG_PRECOMP ? upload_precomp(op, &op->dtab)
-                                                     : upload_device_tables(op->tab, op->d.precision == SFK_FP32, &op->dtab);
+  if (op->dtab && (op->tptab || !has_tp(op))) return SFK_OK;
+  hipError_t e = hipSuccess;
+  if (!op->dtab)
+    e = op->d.kind == SFK_DG_PRECOMP ? upload_precomp(op, &op->dtab)
+                                     : upload_device_tables(op->tab, op->d.precision == SFK_FP32, &op->dtab);
+  if (e == hipSuccess && has_tp(op) && !op->tptab)
+    e = upload_tp_tables(op->tab, op->d.inv_jac, op->d.precision == SFK_FP32, &op->tptab);
   if (e != hipSuccess) {
-    op->dtab = nullptr;
     set_error(std::string("device tables: ") + hipGetErrorString(e));
     return SFK_ERR_HIP;
   }
@@ -113,6 +123,41 @@ sfk_status ensure_scratch(sfk_op *op, size_t bytes)
   }
   op->scratch_bytes = bytes;
   return SFK_OK;
+}
+
+// the continuous kernel a launch of this operator uses: the tensor-product
+// kernel unless the schedule asks for a scatter variant
+bool uses_tp(const sfk_op *op)
+{
+  return has_tp(op) && (op->cont_scatter == 2 || op->cont_scatter == -1);
+}
+
+// default cells in z per workgroup of the tensor-product kernel: about 1024
+// workgroups (4 per CU) unless the mesh has fewer tile columns x cells in z
+int tp_default_zc(const sfk_op *op, int shape)
+{
+  int bx = 0, by = 0;
+  tp_shape(op->d.degree + 1, shape, &bx, &by);
+  if (bx <= 0 || by <= 0) return 0;
+  const long long tiles = (long long)((op->d.n_cells[0] + bx - 1) / bx) * ((op->d.n_cells[1] + by - 1) / by);
+  const int nz = op->d.n_cells[2];
+  long long zc = tiles * nz / 1024;
+  if (zc < 1) zc = 1;
+  if (zc > nz) zc = nz;
+  return (int)zc;
+}
+
+void fill_tp_launch(const sfk_op *op, TpLaunch &L)
+{
+  L.n = op->d.degree + 1;
+  L.fp32 = op->d.precision == SFK_FP32;
+  L.tptab = op->tptab;
+  for (int d = 0; d < 3; ++d) L.nc[d] = op->d.n_cells[d];
+  L.nrhs = op->d.n_rhs;
+  L.vstride = op->nvec;
+  L.shape = op->tp_shape;
+  L.zc = op->tp_zc > 0 ? op->tp_zc : tp_default_zc(op, op->tp_shape);
+  L.order = op->sched_order;
 }
 
 void fill_cell_launch(const sfk_op *op, CellLaunch &L)
@@ -189,6 +234,8 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
     SFK_REQUIRE(d.kind == SFK_DG_PRECOMP, SFK_ERR_UNSUPPORTED,
                 "SFK_FLAG_REF_PRECOMP_COUPLING: precomputed DG Laplacian only");
 
+  const long long ncells = (long long)d.n_cells[0] * d.n_cells[1] * d.n_cells[2];
+  SFK_REQUIRE(ncells <= (1LL << 31), SFK_ERR_INVALID_ARG, "too many cells");
   sfk_op *op = new sfk_op();
   op->d = d;
   // per-q SIPG: slab traversal by default (its re-read neighbour layers and
@@ -200,8 +247,7 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
     return SFK_ERR_INVALID_ARG;
   }
   const long long n = d.degree + 1;
-  op->ncells = (long long)d.n_cells[0] * d.n_cells[1] * d.n_cells[2];
-  SFK_REQUIRE(op->ncells <= (1LL << 31), SFK_ERR_INVALID_ARG, "too many cells");
+  op->ncells = ncells;
   if (d.kind == SFK_CONT_LAPLACE) {
     for (int k = 0; k < 3; ++k) op->grid[k] = d.n_cells[k] * d.degree + 1;
     op->nvec = (long long)op->grid[0] * op->grid[1] * op->grid[2];
@@ -223,7 +269,8 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
 sfk_status sfk_op_destroy(sfk_op *op)
 {
   if (!op) return SFK_OK;
-  for (void *p : {op->cell_metric, op->face_metric, op->coef, op->cell_vel, op->face_vel, op->scratch, op->dtab})
+  for (void *p : {op->cell_metric, op->face_metric, op->coef, op->cell_vel, op->face_vel, op->scratch, op->dtab,
+                  op->tptab})
     if (p) (void)hipFree(p);
   delete op;
   return SFK_OK;
@@ -341,8 +388,16 @@ sfk_status sfk_apply(sfk_op *op, const void *src, void *dst, void *stream)
   case SFK_CELL_LAPLACE:
   case SFK_CONT_LAPLACE: {
     SFK_REQUIRE(!(d.flags & SFK_FLAG_VARIABLE_COEF) || op->coef, SFK_ERR_NOT_READY, "coefficient not set");
-    CellLaunch L;
     SFK_ENSURE_TABLES(op);
+    if (uses_tp(op)) {
+      TpLaunch T;
+      fill_tp_launch(op, T);
+      T.src = src;
+      T.dst = dst;
+      SFK_CHECK_HIP(launch_cont_tp(T, s));
+      return SFK_OK;
+    }
+    CellLaunch L;
     fill_cell_launch(op, L);
     L.mode = d.kind == SFK_CELL_LAPLACE ? 0 : 1;
     L.src = src;
@@ -483,6 +538,22 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
   }
   if (d.kind == SFK_CONT_LAPLACE || d.kind == SFK_CELL_LAPLACE) {
     SFK_REQUIRE(d.n_rhs == 1, SFK_ERR_UNSUPPORTED, "sfk_cheby_step: n_rhs must be 1");
+    SFK_ENSURE_TABLES(op);
+    if (uses_tp(op)) {
+      // the update fused into the tensor-product kernel's stores
+      TpLaunch T;
+      fill_tp_launch(op, T);
+      T.cheby = true;
+      T.src = sol_old;
+      T.dst = sol_new;
+      T.rhs = rhs;
+      T.diag = diag;
+      T.tm = tmp;
+      T.c_np = c_np;
+      T.c_tm = c_tm;
+      SFK_CHECK_HIP(launch_cont_tp(T, s));
+      return SFK_OK;
+    }
     sfk_status st = ensure_scratch(op, (size_t)op->nvec * elem_size(op));
     if (st != SFK_OK) return st;
     st = sfk_apply(op, sol_old, op->scratch, stream);

@@ -406,7 +406,7 @@ cell_kernel(const CellArgs<N, T> a)
         for (int i = 0; i < N; ++i) {
           crhs[i] = a.rhs[g0 + i];
           cdiag[i] = a.diag[g0 + i];
-          ctm[i] = a.tm[g0 + i];
+          ctm[i] = a.c_tm != T(0) ? a.tm[g0 + i] : T(0); // c_tm == 0: tm not read
           csrc[i] = src[g0 + i];
         }
       }
@@ -458,7 +458,8 @@ cell_kernel(const CellArgs<N, T> a)
         for (int i = 0; i < N; ++i) {
           const long long g = g0 + i;
           const T res = o[i] - crhs[i];
-          const T tmv = a.c_tm * ctm[i] + a.c_np * cdiag[i] * res;
+          const T base = a.c_np * cdiag[i] * res;
+          const T tmv = a.c_tm != T(0) ? a.c_tm * ctm[i] + base : base;
           a.tm[g] = tmv;
           dst[g] = csrc[i] - tmv;
         }

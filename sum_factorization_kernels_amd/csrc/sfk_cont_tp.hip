@@ -1,0 +1,499 @@
+// Continuous Q_p Laplacian on an affine Cartesian mesh with a constant
+// coefficient, evaluated as a sum of Kronecker products of the assembled
+// 1D element matrices (tensor-product form).
+//
+// With Gauss quadrature of p+1 points per direction the element mass and
+// stiffness matrices of the GLL-Lagrange basis are integrated exactly, so
+// the cell operator of legacy/include/evaluation_cell_laplacian.h:264-503
+// (value sweeps, collocation gradient, q-point operator G = diag(JxW J_d^2),
+// transposed sweeps) equals
+//   A_cell = c_x K (x) M (x) M + c_y M (x) K (x) M + c_z M (x) M (x) K,
+//   M_ij = sum_q w_q phi_i(x_q) phi_j(x_q),  K_ij = sum_q w_q phi_i'(x_q) phi_j'(x_q),
+//   c_d = JxW J_d^2 (the same constants as the cell kernel's cdiag),
+// and on a structured grid of cells the assembled operator is the same sum
+// with the assembled (banded) 1D matrices M^G, K^G in place of M, K.  The
+// kernel applies it direction by direction on the global node grid,
+//   a = M_y u,  b = c_y K_y u;   c = M_x a,  d = M_x b + c_x K_x a;
+//   A u = M_z d + c_z K_z c,
+// so nodes shared by cells are summed inside each 1D sweep: no scatter,
+// colouring or atomics.  Every 1D sweep is the even-odd contraction of an
+// element line (eo_apply) plus, for the element's first node (shared with
+// the element before it), the last row of the element matrix applied to the
+// element before: outputs of a line are its nodes 0..p-1.  Results equal the
+// quadrature form up to rounding (the test suite pins both against the
+// oracle at 1e-12).  Dirichlet nodes are 0 on input and output.
+//
+// A workgroup owns a tile of TX x TY nodes (BX x BY cells) in x/y and marches
+// through a range of cells in z:
+//   phase Y  (per z layer, the p new node planes): y lines of the input read
+//            from HBM (lanes on consecutive x: coalesced), for the tile's rows
+//            and a halo of p+1 columns (the x sweep needs the cell before the
+//            tile and the tile's last node) -> a, b in LDS;
+//   phase X  x lines of a, b from LDS -> c, d in LDS;
+//   phase Z  one thread per node column: z line of c, d (the layer's first
+//            plane kept in registers from the layer before), z sweep, stores
+//            of the p finished planes (coalesced); the sum for the shared
+//            plane is carried in a register to the next layer.
+// Two barriers per layer.  A z range that starts above z = 0 first rebuilds
+// the carried state from the cell below (one plane, then one layer without
+// stores).  Optionally the Chebyshev update of do_cheby_iter<true>
+// (evaluation_dg_laplacian.h:485-502) is fused into phase Z: the assembled
+// value of a node is final there, and all its streams are coalesced.
+#include "sfk_device.h"
+#include "sfk_kernels.h"
+
+namespace sfk {
+
+// tables of the tensor-product operator (device memory, read through the
+// constant address space like DevTab)
+template <int N, typename T>
+struct TpTab {
+  Eo<N, T> M;    // element mass
+  Eo<N, T> K[3]; // c_d K, d = x, y, z
+  T mrow[N];     // last row of M
+  T krow[3][N];  // last row of c_d K
+};
+
+template <int N, typename T>
+__device__ __forceinline__ const SFK_CONST_AS TpTab<N, T> &tptab(const void *p)
+{
+  return *(const SFK_CONST_AS TpTab<N, T> *)opaque_ptr(p);
+}
+
+constexpr int odd_up(int v) { return v | 1; }
+constexpr int round64(int v) { return (v + 63) / 64 * 64; }
+constexpr int imax(int a, int b) { return a > b ? a : b; }
+
+template <int N, typename T, int BX, int BY>
+struct TpCfg {
+  static constexpr int p = N - 1;
+  static constexpr int TX = BX * p, TY = BY * p;
+  static constexpr int HX = TX + p + 1;   // columns of phase Y (halo of p before, 1 after)
+  static constexpr int RPA = odd_up(HX);  // row pitches: odd, so that 16 lanes on
+  static constexpr int RPC = odd_up(TX);  // consecutive rows hit distinct banks
+  static constexpr int NYI = p * BY * HX; // work items of phase Y (p planes)
+  static constexpr int NXI = p * TY * BX; // phase X
+  static constexpr int NZI = TX * TY;     // phase Z (node columns)
+  static constexpr int THREADS_RAW = round64(imax(NYI, imax(NXI, NZI)));
+  static constexpr int THREADS = THREADS_RAW > 1024 ? 1024 : THREADS_RAW;
+  static constexpr int LA = p * TY * RPA; // one of a, b
+  static constexpr int LC = p * TY * RPC; // one of c, d
+  static constexpr int LDS_VALUES = 2 * LA + 2 * LC;
+};
+
+template <typename T>
+struct TpArgs {
+  const void *tab;
+  const T *src;
+  T *dst;
+  const T *rhs, *diag; // fused Chebyshev
+  T *tm;
+  T c_np, c_tm;
+  int nx, ny, nz;    // cells
+  int gx, gy, gz;    // nodes
+  int ntx, nty;      // tiles per direction
+  int zc;            // cells in z per workgroup
+  int order;         // block order (logical_block)
+  long long vstride; // elements between right-hand sides
+};
+
+// element line with the shared first node: in[0..2p] = values at nodes
+// e p - p .. e p + p; out[i] = (A u_e)[i] + (i == 0 ? (A u_{e-1})[p] : 0),
+// i < p, for A = M (oa) and A = K (ob)
+template <int N, typename T, typename TB>
+__device__ __forceinline__ void line_mk(const TB &tb, int dir, const T (&in)[2 * N - 1], T (&oa)[N - 1],
+                                        T (&ob)[N - 1])
+{
+  constexpr int p = N - 1;
+  T ue[N], ra[N], rb[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) ue[i] = in[p + i];
+  eo_apply<N, 1>(tb.M, ue, ra);
+  eo_apply<N, 1>(tb.K[dir], ue, rb);
+  T va = tb.mrow[0] * in[0], vb = tb.krow[dir][0] * in[0];
+#pragma unroll
+  for (int j = 1; j < N; ++j) {
+    va = tb.mrow[j] * in[j] + va;
+    vb = tb.krow[dir][j] * in[j] + vb;
+  }
+  oa[0] = ra[0] + va;
+  ob[0] = rb[0] + vb;
+#pragma unroll
+  for (int i = 1; i < p; ++i) {
+    oa[i] = ra[i];
+    ob[i] = rb[i];
+  }
+}
+
+template <int N, typename T, int BX, int BY, bool CHEBY>
+__global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel(const TpArgs<T> a)
+{
+  using C = TpCfg<N, T, BX, BY>;
+  constexpr int p = N - 1, TX = C::TX, TY = C::TY, HX = C::HX, RPA = C::RPA, RPC = C::RPC;
+  T *sA = dyn_lds<T>();
+  T *sB = sA + C::LA;
+  T *sC = sB + C::LA;
+  T *sD = sC + C::LC;
+
+  const unsigned L = logical_block(blockIdx.x, gridDim.x, a.order);
+  const int tx = (int)(L % (unsigned)a.ntx);
+  const unsigned r0 = L / (unsigned)a.ntx;
+  const int ty = (int)(r0 % (unsigned)a.nty);
+  const int chunk = (int)(r0 / (unsigned)a.nty);
+  const int ex0 = tx * BX, ey0 = ty * BY;
+  const int x0 = ex0 * p, y0 = ey0 * p;
+  const int ez_begin = chunk * a.zc;
+  const int ez_end = ez_begin + a.zc < a.nz ? ez_begin + a.zc : a.nz;
+  const int gx = a.gx, gy = a.gy, gz = a.gz;
+  const long long plane = (long long)gx * gy;
+  const long long rr = blockIdx.y;
+  const T *src = a.src + rr * a.vstride;
+  T *dst = a.dst + rr * a.vstride;
+
+  // phase Y, split in two so that the loads of the next layer are in flight
+  // during phases X and Z of the current one: load_y fetches this thread's
+  // y line (planes Z = ez p + k0 + s, s < NK, one work item per thread:
+  // NK BY HX <= THREADS) into registers, store_y contracts it into slot s.
+  static_assert(p * BY * HX <= C::THREADS, "one phase-Y item per thread");
+  T u[2 * N - 1];
+  bool yact = false;
+  auto load_y = [&](int ez, int k0, auto nkc) {
+    constexpr int NK = decltype(nkc)::value;
+    const int it = threadIdx.x;
+    const int xh = it % HX, rest = it / HX, eyl = rest % BY, s = rest / BY;
+    const int ey = ey0 + eyl;
+    // rows beyond the mesh are never read
+    yact = it < NK * BY * HX && ey < a.ny;
+    const int X = x0 - p + xh, Z = ez * p + k0 + s;
+    if (yact && X > 0 && X < gx - 1 && Z > 0 && Z < gz - 1) {
+      const T *col = src + (long long)Z * plane + X;
+#pragma unroll
+      for (int j = 0; j < 2 * N - 1; ++j) {
+        const int Y = ey * p - p + j;
+        u[j] = (Y > 0 && Y < gy - 1) ? col[(long long)Y * gx] : T(0);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 2 * N - 1; ++j) u[j] = T(0);
+    }
+  };
+  auto store_y = [&]() {
+    if (!yact) return;
+    const auto &tb = tptab<N, T>(a.tab);
+    const int it = threadIdx.x;
+    const int xh = it % HX, rest = it / HX, eyl = rest % BY, s = rest / BY;
+    T va[N - 1], vb[N - 1];
+    line_mk<N, T>(tb, 1, u, va, vb);
+    T *ra = sA + (s * TY + eyl * p) * RPA + xh;
+    T *rb = sB + (s * TY + eyl * p) * RPA + xh;
+#pragma unroll
+    for (int i = 0; i < p; ++i) {
+      ra[i * RPA] = va[i];
+      rb[i * RPA] = vb[i];
+    }
+  };
+
+  // phase X: rows of slots s < NK
+  auto phase_x = [&](auto nkc) {
+    constexpr int NK = decltype(nkc)::value;
+    const auto &tb = tptab<N, T>(a.tab);
+    for (int it = threadIdx.x; it < NK * TY * BX; it += C::THREADS) {
+      const int yt = it % TY, rest = it / TY, s = rest % NK, exl = rest / NK;
+      if (ex0 + exl >= a.nx || y0 + yt >= gy - 1) continue; // no owned output
+      const T *ra = sA + (s * TY + yt) * RPA + exl * p;
+      const T *rb = sB + (s * TY + yt) * RPA + exl * p;
+      T ua[2 * N - 1], ub[2 * N - 1];
+#pragma unroll
+      for (int j = 0; j < 2 * N - 1; ++j) {
+        ua[j] = ra[j];
+        ub[j] = rb[j];
+      }
+      // c = M a;  d = M b + c_x K a (K b, also formed by line_mk, is unused
+      // and dropped by the compiler)
+      T cm[N - 1], ck[N - 1], dm[N - 1], dk[N - 1];
+      line_mk<N, T>(tb, 0, ua, cm, ck);
+      line_mk<N, T>(tb, 0, ub, dm, dk);
+      T *rc = sC + (s * TY + yt) * RPC + exl * p;
+      T *rd = sD + (s * TY + yt) * RPC + exl * p;
+#pragma unroll
+      for (int i = 0; i < p; ++i) {
+        rc[i] = cm[i];
+        rd[i] = dm[i] + ck[i];
+      }
+    }
+  };
+
+  // phase Z state of this thread's node column
+  const int zt = threadIdx.x;
+  const int Xt = zt % TX, Yt = zt / TX;
+  const int X = x0 + Xt, Y = y0 + Yt;
+  // 0: compute, 1: Dirichlet column of the grid's last x or y node (zeros),
+  // 2: outside the grid or no column
+  const int zone = zt >= C::NZI || X >= gx || Y >= gy ? 2 : (X == gx - 1 || Y == gy - 1 ? 1 : 0);
+  // the tile ends exactly at the last (Dirichlet) node: its zeros are
+  // written by the threads of the nodes before it
+  const bool xtra_x = zone != 2 && Xt == TX - 1 && X + 1 == gx - 1;
+  const bool xtra_y = zone != 2 && Yt == TY - 1 && Y + 1 == gy - 1;
+  const long long colg = (long long)Y * gx + X;
+  T cp = T(0), dp = T(0), yp = T(0);
+
+  auto put = [&](long long g, T v) {
+    if constexpr (CHEBY) {
+      // r = A u - rhs; tm = c_tm tm + c_np diag r; dst = u - tm
+      const T res = v - a.rhs[g];
+      const T t = (a.c_tm != T(0) ? a.c_tm * a.tm[g] : T(0)) + a.c_np * a.diag[g] * res;
+      a.tm[g] = t;
+      dst[g] = src[g] - t;
+    } else {
+      __builtin_nontemporal_store(v, dst + g);
+    }
+  };
+  // zero (Dirichlet) outputs of plane Z for this thread: its own node when
+  // zone 1, the extra last-node columns
+  auto put_zeros = [&](int Z) {
+    const long long g = (long long)Z * plane + colg;
+    if (zone == 1) put(g, T(0));
+    if (xtra_x) put(g + 1, T(0));
+    if (xtra_y) put(g + gx, T(0));
+    if (xtra_x && xtra_y) put(g + gx + 1, T(0));
+  };
+
+  auto phase_z = [&](int ez, bool store) {
+    if (zone == 0) {
+      const auto &tb = tptab<N, T>(a.tab);
+      T c[N], d[N], o[N];
+      c[0] = cp;
+      d[0] = dp;
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        c[k] = sC[((k - 1) * TY + Yt) * RPC + Xt];
+        d[k] = sD[((k - 1) * TY + Yt) * RPC + Xt];
+      }
+      T om[N], ok[N];
+      eo_apply<N, 1>(tb.M, d, om);
+      eo_apply<N, 1>(tb.K[2], c, ok);
+#pragma unroll
+      for (int k = 0; k < N; ++k) o[k] = om[k] + ok[k];
+      o[0] += yp;
+      if (store) {
+        const bool bxy = X == 0 || Y == 0;
+#pragma unroll
+        for (int k = 0; k < p; ++k) {
+          const int Z = ez * p + k;
+          put((long long)Z * plane + colg, (bxy || Z == 0) ? T(0) : o[k]);
+        }
+      }
+      cp = c[p];
+      dp = d[p];
+      yp = o[p];
+    }
+    if (store && zone != 2)
+      for (int k = 0; k < p; ++k) put_zeros(ez * p + k);
+  };
+
+  using one = std::integral_constant<int, 1>;
+  using all = std::integral_constant<int, p>;
+  // one layer: phase Y from the prefetched registers, the next layer's
+  // loads, phases X and Z
+  auto layer = [&](int ez, bool store, int ez_next) {
+    store_y();
+    __syncthreads();
+    if (ez_next >= 0) load_y(ez_next, 1, all());
+    phase_x(all());
+    __syncthreads();
+    phase_z(ez, store);
+  };
+  if (ez_begin > 0) {
+    // carried state at plane ez_begin p: c, d of the plane below the cell
+    // below, then that cell's layer without stores
+    load_y(ez_begin - 1, 0, one());
+    store_y();
+    __syncthreads();
+    phase_x(one());
+    __syncthreads();
+    if (zone == 0) {
+      cp = sC[Yt * RPC + Xt];
+      dp = sD[Yt * RPC + Xt];
+    }
+    load_y(ez_begin - 1, 1, all());
+    layer(ez_begin - 1, false, ez_begin < ez_end ? ez_begin : -1);
+  } else if (ez_begin < ez_end) {
+    load_y(ez_begin, 1, all());
+  }
+  for (int ez = ez_begin; ez < ez_end; ++ez) layer(ez, true, ez + 1 < ez_end ? ez + 1 : -1);
+  // the grid's last z plane (Dirichlet)
+  if (ez_end == a.nz && zone != 2) {
+    const int Z = gz - 1;
+    if (zone == 0) put((long long)Z * plane + colg, T(0));
+    put_zeros(Z);
+  }
+}
+
+// ------------------------------------------------------------------ host
+
+template <int N, typename T>
+static TpTab<N, T> make_tptab(const Tables1D &t, const double inv_jac[3])
+{
+  // element matrices on [0,1] from the basis values S[i][q] and the basis
+  // derivatives G = S D at the Gauss points
+  std::vector<double> M(N * N, 0.), K(N * N, 0.), G(N * N, 0.);
+  for (int i = 0; i < N; ++i)
+    for (int q = 0; q < N; ++q) {
+      double g = 0.;
+      for (int m = 0; m < N; ++m) g += t.S[i * N + m] * t.D[m * N + q];
+      G[i * N + q] = g;
+    }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      double m = 0., k = 0.;
+      for (int q = 0; q < N; ++q) {
+        m += t.weights[q] * t.S[i * N + q] * t.S[j * N + q];
+        k += t.weights[q] * G[i * N + q] * G[j * N + q];
+      }
+      M[i * N + j] = m;
+      K[i * N + j] = k;
+    }
+  // symmetrize (exact in exact arithmetic) and impose the centro-symmetry
+  // the even-odd form relies on
+  auto sym = [&](std::vector<double> &A) {
+    std::vector<double> B(N * N);
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j)
+        B[i * N + j] = 0.25 * (A[i * N + j] + A[j * N + i] + A[(N - 1 - i) * N + (N - 1 - j)] +
+                               A[(N - 1 - j) * N + (N - 1 - i)]);
+    A = B;
+  };
+  sym(M);
+  sym(K);
+  const double jxw = 1. / (inv_jac[0] * inv_jac[1] * inv_jac[2]);
+  TpTab<N, T> tb{};
+  tb.M = make_eo<N, T>(M, true);
+  for (int d = 0; d < 3; ++d) {
+    const double c = inv_jac[d] * inv_jac[d] * jxw;
+    std::vector<double> Kd(N * N);
+    for (int i = 0; i < N * N; ++i) Kd[i] = c * K[i];
+    tb.K[d] = make_eo<N, T>(Kd, true);
+    for (int j = 0; j < N; ++j) tb.krow[d][j] = T(Kd[(N - 1) * N + j]);
+  }
+  for (int j = 0; j < N; ++j) tb.mrow[j] = T(M[(N - 1) * N + j]);
+  return tb;
+}
+
+template <int N, typename T>
+static hipError_t upload_tp(const Tables1D &t, const double inv_jac[3], void **dev)
+{
+  const TpTab<N, T> h = make_tptab<N, T>(t, inv_jac);
+  hipError_t e = hipMalloc(dev, sizeof(h));
+  if (e != hipSuccess) {
+    *dev = nullptr;
+    return e;
+  }
+  e = hipMemcpy(*dev, &h, sizeof(h), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(*dev);
+    *dev = nullptr;
+  }
+  return e;
+}
+
+hipError_t upload_tp_tables(const Tables1D &t, const double inv_jac[3], bool fp32, void **dev)
+{
+  switch (t.n) {
+#define SFK_CASE(NN) \
+  case NN: return fp32 ? upload_tp<NN, float>(t, inv_jac, dev) : upload_tp<NN, double>(t, inv_jac, dev);
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+#undef SFK_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N, typename T, int BX, int BY>
+static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
+{
+  using C = TpCfg<N, T, BX, BY>;
+  constexpr int p = N - 1;
+  TpArgs<T> a{};
+  a.tab = L.tptab;
+  a.src = (const T *)L.src;
+  a.dst = (T *)L.dst;
+  a.rhs = (const T *)L.rhs;
+  a.diag = (const T *)L.diag;
+  a.tm = (T *)L.tm;
+  a.c_np = T(L.c_np);
+  a.c_tm = T(L.c_tm);
+  a.nx = L.nc[0];
+  a.ny = L.nc[1];
+  a.nz = L.nc[2];
+  a.gx = a.nx * p + 1;
+  a.gy = a.ny * p + 1;
+  a.gz = a.nz * p + 1;
+  a.ntx = (a.nx + BX - 1) / BX;
+  a.nty = (a.ny + BY - 1) / BY;
+  a.zc = L.zc > 0 ? (L.zc < a.nz ? L.zc : a.nz) : a.nz;
+  a.order = L.order;
+  a.vstride = L.vstride;
+  const long long nchunks = (a.nz + a.zc - 1) / a.zc;
+  const long long blocks = (long long)a.ntx * a.nty * nchunks;
+  if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  const size_t lds = size_t(C::LDS_VALUES) * sizeof(T);
+  auto kern = L.cheby ? cont_tp_kernel<N, T, BX, BY, true> : cont_tp_kernel<N, T, BX, BY, false>;
+  hipError_t e = allow_dynamic_lds((const void *)kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks, L.nrhs), dim3(C::THREADS), lds, s, a);
+  return hipGetLastError();
+}
+
+// tile shapes (BX, BY) per points per direction N; shape 0 is the default
+#define SFK_TP_SHAPES(X)                                                                       \
+  X(2, 0, 32, 16) X(2, 1, 64, 8) X(2, 2, 16, 32)                                               \
+  X(3, 0, 16, 8) X(3, 1, 32, 4) X(3, 2, 8, 16)                                                 \
+  X(4, 0, 8, 8) X(4, 1, 16, 4) X(4, 2, 4, 16)                                                  \
+  X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 4, 8)                                                   \
+  X(6, 0, 4, 4) X(6, 1, 8, 2) X(6, 2, 8, 4)                                                    \
+  X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2)                                                    \
+  X(8, 0, 4, 2) X(8, 1, 2, 4) X(8, 2, 2, 2)                                                    \
+  X(9, 0, 4, 2) X(9, 1, 2, 2) X(9, 2, 4, 1)
+
+constexpr int kTpShapes = 3;
+
+int tp_shape_count(int) { return kTpShapes; }
+
+void tp_shape(int n, int shape, int *bx, int *by)
+{
+  *bx = *by = 0;
+#define SFK_X(NN, SH, BXX, BYY) \
+  if (n == NN && shape == SH) { \
+    *bx = BXX;                  \
+    *by = BYY;                  \
+  }
+  SFK_TP_SHAPES(SFK_X)
+#undef SFK_X
+}
+
+size_t tp_lds_bytes(int n, int shape, bool fp32)
+{
+  size_t v = 0;
+#define SFK_X(NN, SH, BXX, BYY) \
+  if (n == NN && shape == SH) v = (size_t)TpCfg<NN, double, BXX, BYY>::LDS_VALUES;
+  SFK_TP_SHAPES(SFK_X)
+#undef SFK_X
+  return v * (fp32 ? 4 : 8);
+}
+
+template <typename T>
+static hipError_t launch_tp_t(const TpLaunch &L, hipStream_t s)
+{
+  const int shape = L.shape >= 0 && L.shape < kTpShapes ? L.shape : 0;
+#define SFK_X(NN, SH, BXX, BYY) \
+  if (L.n == NN && shape == SH) return launch_tp_shape<NN, T, BXX, BYY>(L, s);
+  SFK_TP_SHAPES(SFK_X)
+#undef SFK_X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_cont_tp(const TpLaunch &L, hipStream_t s)
+{
+  return L.fp32 ? launch_tp_t<float>(L, s) : launch_tp_t<double>(L, s);
+}
+
+} // namespace sfk

@@ -44,6 +44,31 @@ struct CellLaunch {
 };
 
 hipError_t launch_cell(const CellLaunch &L, hipStream_t stream);
+
+// continuous Laplacian, constant coefficient: tensor-product (Kronecker
+// sum) kernel on the node grid (sfk_cont_tp.hip)
+struct TpLaunch {
+  int n = 0;            // points per direction (degree + 1)
+  bool fp32 = false;
+  bool cheby = false;   // fused Chebyshev update
+  const void *tptab = nullptr; // TpTab<n, T> in device memory
+  const void *src = nullptr;
+  void *dst = nullptr;
+  const void *rhs = nullptr, *diag = nullptr;
+  void *tm = nullptr;
+  double c_np = 0., c_tm = 0.;
+  int nc[3] = {1, 1, 1};
+  int nrhs = 1;
+  long long vstride = 0;
+  int shape = 0;        // tile shape index (tp_shape)
+  int zc = 0;           // cells in z per workgroup (0: all)
+  int order = 1;        // block order
+};
+hipError_t launch_cont_tp(const TpLaunch &L, hipStream_t stream);
+hipError_t upload_tp_tables(const Tables1D &t, const double inv_jac[3], bool fp32, void **dev);
+int tp_shape_count(int n);
+void tp_shape(int n, int shape, int *bx, int *by);
+size_t tp_lds_bytes(int n, int shape, bool fp32);
 hipError_t launch_cell_diagonal(const CellLaunch &L, hipStream_t stream);
 
 // DG-SIPG with per-quadrature-point metric (deformed mesh)

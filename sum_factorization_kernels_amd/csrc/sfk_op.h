@@ -20,8 +20,12 @@ struct sfk_op {
   size_t scratch_bytes = 0;
   void *dtab = nullptr; // 1D tables in device memory (uploaded on first launch)
   // workgroup schedule (sfk_schedule): cells per block (0 = default), block
-  // order, continuous scatter (-1 automatic, 0 atomic, 1 colored)
+  // order, continuous scatter (-1 automatic, 0 atomic, 1 colored, 2
+  // tensor-product kernel) and the tensor-product kernel's tile shape and
+  // cells in z per workgroup (0 = automatic)
   int sched_cb = 0, sched_order = 1, cont_scatter = -1;
+  int tp_shape = 0, tp_zc = 0;
+  void *tptab = nullptr; // TpTab of the tensor-product kernel (constant coefficient)
   // precomputed DG Laplacian: neighbour coupling constants not both zero
   // (set with the device tables)
   bool pc_couple = true;

@@ -394,7 +394,7 @@ sipg_general_kernel(const GenArgs<N, T> a)
       for (int i = 0; i < N; ++i) {
         crhs[i] = a.rhs[g0 + i];
         cdiag[i] = a.diag[g0 + i];
-        ctm[i] = a.tm[g0 + i];
+        ctm[i] = a.c_tm != T(0) ? a.tm[g0 + i] : T(0); // c_tm == 0: tm not read
         csrc[i] = a.src[g0 + i];
       }
     }
@@ -420,7 +420,8 @@ sipg_general_kernel(const GenArgs<N, T> a)
         for (int i = 0; i < N; ++i) {
           const long long g = g0 + i;
           const T res = o[i] - crhs[i];
-          const T tmv = a.c_tm * ctm[i] + a.c_np * cdiag[i] * res;
+          const T base = a.c_np * cdiag[i] * res;
+          const T tmv = a.c_tm != T(0) ? a.c_tm * ctm[i] + base : base;
           a.tm[g] = tmv;
           a.dst[g] = csrc[i] - tmv;
         }

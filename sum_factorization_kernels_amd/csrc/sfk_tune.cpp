@@ -50,15 +50,28 @@ size_t lds_per_cell(const sfk_op *op)
   }
 }
 
-int max_cb(const sfk_op *op)
+// LDS per workgroup of the current device (gfx950: 160 KiB when no device
+// can be queried)
+int lds_limit()
 {
-  const int n = op->d.degree + 1;
-  int dev = 0, lds_max = 65536;
+  int dev = 0, lds_max = 163840;
   if (hipGetDevice(&dev) == hipSuccess) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
       lds_max = v;
   }
+  return lds_max;
+}
+
+bool tp_ok(const sfk_op *op)
+{
+  return op->d.kind == SFK_CONT_LAPLACE && !(op->d.flags & SFK_FLAG_VARIABLE_COEF);
+}
+
+int max_cb(const sfk_op *op)
+{
+  const int n = op->d.degree + 1;
+  const int lds_max = lds_limit();
   int m = kMaxThreads / (n * n);
   const int by_lds = (int)(size_t(lds_max) / lds_per_cell(op));
   if (by_lds < m) m = by_lds;
@@ -75,6 +88,21 @@ sfk_status sfk_op_get_schedule(const sfk_op *op, sfk_schedule *s)
   s->cells_per_block = op->sched_cb > 0 ? op->sched_cb : default_cb(op);
   s->block_order = op->sched_order;
   s->cont_scatter = op->cont_scatter;
+  s->tile_shape = op->tp_shape;
+  s->z_cells = op->tp_zc;
+  return SFK_OK;
+}
+
+sfk_status sfk_op_tile_shape(const sfk_op *op, int shape, int *count, int *bx, int *by)
+{
+  if (!op) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_op_tile_shape: null handle");
+  const int n = op->d.degree + 1, c = tp_shape_count(n);
+  if (count) *count = c;
+  if (shape < 0 || shape >= c) TUNE_FAIL(SFK_ERR_INVALID_ARG, "tile shape must be in 0.." + std::to_string(c - 1));
+  int x = 0, y = 0;
+  tp_shape(n, shape, &x, &y);
+  if (bx) *bx = x;
+  if (by) *by = y;
   return SFK_OK;
 }
 
@@ -92,10 +120,23 @@ sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s)
   if (s->cells_per_block < 0 || s->cells_per_block > m)
     TUNE_FAIL(SFK_ERR_INVALID_ARG, "cells_per_block must be in 0.." + std::to_string(m));
   if (s->block_order < 0 || s->block_order > 2) TUNE_FAIL(SFK_ERR_INVALID_ARG, "block_order must be 0, 1 or 2");
-  if (s->cont_scatter < -1 || s->cont_scatter > 1) TUNE_FAIL(SFK_ERR_INVALID_ARG, "cont_scatter must be -1, 0 or 1");
+  if (s->cont_scatter < -1 || s->cont_scatter > 2)
+    TUNE_FAIL(SFK_ERR_INVALID_ARG, "cont_scatter must be -1, 0, 1 or 2");
+  if (s->cont_scatter == 2 && !tp_ok(op))
+    TUNE_FAIL(SFK_ERR_UNSUPPORTED, "cont_scatter 2: continuous Laplacian with a constant coefficient only");
+  if (s->tile_shape < 0 || s->tile_shape >= tp_shape_count(op->d.degree + 1))
+    TUNE_FAIL(SFK_ERR_INVALID_ARG, "tile_shape out of range");
+  if (s->z_cells < 0) TUNE_FAIL(SFK_ERR_INVALID_ARG, "z_cells must be >= 0");
+  if (s->cont_scatter == 2 || s->cont_scatter == -1) {
+    const size_t need = tp_lds_bytes(op->d.degree + 1, s->tile_shape, op->d.precision == SFK_FP32);
+    if (tp_ok(op) && need > (size_t)lds_limit())
+      TUNE_FAIL(SFK_ERR_UNSUPPORTED, "tile shape needs more LDS than the device has per workgroup");
+  }
   op->sched_cb = s->cells_per_block;
   op->sched_order = s->block_order;
   op->cont_scatter = s->cont_scatter;
+  op->tp_shape = s->tile_shape;
+  op->tp_zc = s->z_cells;
   return SFK_OK;
 }
 
@@ -109,7 +150,8 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
   hipEvent_t e0 = nullptr, e1 = nullptr;
   sfk_status status = SFK_OK;
   const int saved_cb = op->sched_cb, saved_order = op->sched_order, saved_scatter = op->cont_scatter;
-  sfk_schedule win{saved_cb > 0 ? saved_cb : default_cb(op), saved_order, saved_scatter};
+  const int saved_shape = op->tp_shape, saved_zc = op->tp_zc;
+  sfk_schedule win{saved_cb > 0 ? saved_cb : default_cb(op), saved_order, saved_scatter, saved_shape, saved_zc};
   double win_t = -1.;
   if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
     set_error("sfk_tune: scratch allocation failed");
@@ -121,6 +163,26 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
   }
   if (status == SFK_OK)
     status = sfk_vec_fill(op->d.precision, x, op->nvec * op->d.n_rhs, 1.0, stream);
+  // time one candidate (already set on the handle)
+  auto trial = [&](const sfk_schedule &cand) {
+    status = sfk_apply(op, x, y, stream); // warm-up (and LDS attribute)
+    if (status != SFK_OK) return;
+    if (hipEventRecord(e0, s) != hipSuccess) status = SFK_ERR_HIP;
+    for (int r = 0; r < repeats && status == SFK_OK; ++r) status = sfk_apply(op, x, y, stream);
+    if (status != SFK_OK) return;
+    float ms = 0.f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      set_error("sfk_tune: event timing failed");
+      status = SFK_ERR_HIP;
+      return;
+    }
+    const double t = 1e-3 * ms / repeats;
+    if (win_t < 0. || t < win_t) {
+      win_t = t;
+      win = cand;
+    }
+  };
   if (status == SFK_OK) {
     const int m = max_cb(op);
     std::vector<int> scatters = {op->cont_scatter};
@@ -133,24 +195,28 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
           op->sched_cb = cb;
           op->sched_order = order;
           op->cont_scatter = sc;
-          status = sfk_apply(op, x, y, stream); // warm-up (and LDS attribute)
-          if (status != SFK_OK) break;
-          if (hipEventRecord(e0, s) != hipSuccess) status = SFK_ERR_HIP;
-          for (int r = 0; r < repeats && status == SFK_OK; ++r) status = sfk_apply(op, x, y, stream);
-          if (status != SFK_OK) break;
-          float ms = 0.f;
-          if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            set_error("sfk_tune: event timing failed");
-            status = SFK_ERR_HIP;
-            break;
-          }
-          const double t = 1e-3 * ms / repeats;
-          if (win_t < 0. || t < win_t) {
-            win_t = t;
-            win = sfk_schedule{cb, order, sc};
-          }
+          trial(sfk_schedule{cb, order, sc, saved_shape, saved_zc});
         }
+    if (tp_ok(op)) {
+      // tensor-product kernel: tile shapes x cells in z per workgroup
+      const int n = op->d.degree + 1, nz = op->d.n_cells[2];
+      const int lds_max = lds_limit();
+      std::vector<int> zcs;
+      for (int z = 1; z < nz; z *= 2) zcs.push_back(z);
+      zcs.push_back(nz);
+      for (int sh = 0; sh < tp_shape_count(n) && status == SFK_OK; ++sh) {
+        if (tp_lds_bytes(n, sh, op->d.precision == SFK_FP32) > (size_t)lds_max) continue;
+        for (int order = 0; order <= 1 && status == SFK_OK; ++order)
+          for (int zc : zcs) {
+            if (status != SFK_OK) break;
+            op->sched_order = order;
+            op->cont_scatter = 2;
+            op->tp_shape = sh;
+            op->tp_zc = zc;
+            trial(sfk_schedule{saved_cb, order, 2, sh, zc});
+          }
+      }
+    }
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -160,11 +226,15 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
     op->sched_cb = saved_cb;
     op->sched_order = saved_order;
     op->cont_scatter = saved_scatter;
+    op->tp_shape = saved_shape;
+    op->tp_zc = saved_zc;
     return status != SFK_OK ? status : SFK_ERR_HIP;
   }
   op->sched_cb = win.cells_per_block;
   op->sched_order = win.block_order;
   op->cont_scatter = win.cont_scatter;
+  op->tp_shape = win.tile_shape;
+  op->tp_zc = win.z_cells;
   if (best) *best = win;
   if (best_seconds) *best_seconds = win_t;
   return SFK_OK;

@@ -102,12 +102,15 @@ def test_schedule_defaults_and_validation(degree, kind, geom, default_cb):
         kw["velocity"] = (1.0, 0.15, -0.05)
     op = sfk.Operator(kind, degree, (4, 4, 4), geometry=geom, inv_jac=(4, 4, 4), **kw)
     # the per-q SIPG kernel walks slabs by default (block order 2)
-    assert op.schedule == (default_cb, 2 if geom == sfk.GEOM_PER_Q else 1, -1)
+    assert op.schedule == (default_cb, 2 if geom == sfk.GEOM_PER_Q else 1, -1, 0, 0)
     m = op.max_cells_per_block
     assert default_cb <= m <= 256 // (degree + 1) ** 2
     op.set_schedule(1, 0, 1)
-    assert op.schedule == (1, 0, 1)
-    for bad in [(m + 1, 1, -1), (-1, 1, -1), (0, 3, -1), (0, 1, 2)]:
+    assert op.schedule == (1, 0, 1, 0, 0)
+    # scatter 2 (tensor-product kernel) exists for the continuous Laplacian only
+    bad_scatter = 3 if kind == sfk.CONT_LAPLACE else 2
+    for bad in [(m + 1, 1, -1), (-1, 1, -1), (0, 3, -1), (0, 1, bad_scatter), (0, 1, -1, -1, 0),
+                (0, 1, -1, 99, 0), (0, 1, -1, 0, -1)]:
         with pytest.raises(sfk.SfkError):
             op.set_schedule(*bad)
     with pytest.raises(sfk.SfkError):

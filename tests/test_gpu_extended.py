@@ -37,6 +37,12 @@ def test_sipg_general_matches_oracle(cuda, degree, prec):
         r_tm = 0.45 * tm + 0.6 * diag * (ref - rhs)
         assert rel_err(to_np(d_tm), r_tm) < TOL_FP64
         assert rel_err(to_np(d_new), src - r_tm) < TOL_FP64
+        # c_tm == 0: tmp is not read (NaN in it must not leak)
+        d_new, d_tm = to_dev(np.zeros(N)), torch.full((N,), float("nan"), dtype=dt, device="cuda")
+        op.cheby_step(to_dev(rhs), to_dev(src), to_dev(diag), d_new, d_tm, 0.6, 0.0)
+        r_tm = 0.6 * diag * (ref - rhs)
+        assert rel_err(to_np(d_tm), r_tm) < TOL_FP64
+        assert rel_err(to_np(d_new), src - r_tm) < TOL_FP64
 
 
 @pytest.mark.parametrize("degree", [1, 2, 3, 5])

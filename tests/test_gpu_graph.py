@@ -32,7 +32,7 @@ def _sipg_op(degree, nc):
     return op
 
 
-@pytest.mark.parametrize("kind,degree", [("sipg", 3), ("sipg", 5), ("cont", 4)])
+@pytest.mark.parametrize("kind,degree", [("sipg", 3), ("sipg", 5), ("cont", 4), ("cont_tp", 4), ("cont_tp", 6)])
 @pytest.mark.parametrize("cheby_degree", [1, 4, 5])
 def test_cheby_sweep_equals_fused_steps(cuda, kind, degree, cheby_degree):
     import torch
@@ -41,7 +41,9 @@ def test_cheby_sweep_equals_fused_steps(cuda, kind, degree, cheby_degree):
         op = _sipg_op(degree, nc)
     else:
         op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=(3.0, 2.0, 4.0))
-        op.set_schedule(0, 1, 1)  # colored scatter: deterministic sums (atomics are not)
+        # colored scatter: deterministic sums (atomics are not); the
+        # tensor-product kernel (fused update) is deterministic as well
+        op.set_schedule(0, 1, 1 if kind == "cont" else 2, 0, 1)
     rng = np.random.default_rng(6000 + degree)
     N = op.vector_size
     diag = op.diagonal()
@@ -51,11 +53,11 @@ def test_cheby_sweep_equals_fused_steps(cuda, kind, degree, cheby_degree):
         rhs = op.rhs_constant(1.0)
     x0 = to_dev(rng.standard_normal(N)) if kind == "sipg" else torch.zeros(N, dtype=torch.float64, device="cuda")
     lmin, lmax = 0.05, 2.5
-    # the sweep
-    x, spare, tmp = x0.clone(), torch.empty_like(x0), torch.zeros_like(x0)
+    # the sweep (the work vector's initial contents are never read: NaN)
+    x, spare, tmp = x0.clone(), torch.empty_like(x0), torch.full_like(x0, float("nan"))
     op.cheby_sweep(rhs, x, spare, dinv, tmp, cheby_degree, lmin, lmax)
     # the same steps one by one
-    cur, nxt, tm = x0.clone(), torch.empty_like(x0), torch.zeros_like(x0)
+    cur, nxt, tm = x0.clone(), torch.empty_like(x0), torch.full_like(x0, float("nan"))
     for c_np, c_tm in _cheby_coeffs(cheby_degree, lmin, lmax):
         op.cheby_step(rhs, cur, dinv, nxt, tm, c_np, c_tm)
         cur, nxt = nxt, cur
@@ -67,6 +69,14 @@ def test_cheby_sweep_equals_fused_steps(cuda, kind, degree, cheby_degree):
         cm, fm = sine_metric(degree, nc, 0.1)
         for c_np, c_tm in _cheby_coeffs(cheby_degree, lmin, lmax):
             tr = c_tm * tr + c_np * to_np(dinv) * (ob.dg_sipg_general(degree, nc, cm, fm, xr) - to_np(rhs))
+            xr = xr - tr
+        assert rel_err(to_np(x), xr) < 1e-11
+    else:
+        # against the oracle's operator with the same recurrence
+        xr, tr = to_np(x0), np.zeros(N)
+        ij = (3.0, 2.0, 4.0)
+        for c_np, c_tm in _cheby_coeffs(cheby_degree, lmin, lmax):
+            tr = c_tm * tr + c_np * to_np(dinv) * (ob.cont_laplacian(degree, nc, ij, xr.copy()) - to_np(rhs))
             xr = xr - tr
         assert rel_err(to_np(x), xr) < 1e-11
 
@@ -95,7 +105,7 @@ def test_graph_cheby_sweep_replays(cuda):
         op.cheby_sweep(rhs, x, spare, dinv, tmp, 5, 0.1, 2.0, stream=st.ptr)
     for rep in range(4):
         x.copy_(x0)  # eager work between replays
-        tmp.zero_()
+        tmp.fill_(float("nan"))  # never read: the sweep's first step has c_tm = 0
         torch.cuda.synchronize()
         g.launch()
         st.synchronize()

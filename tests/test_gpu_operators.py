@@ -62,6 +62,16 @@ def test_dg_sipg_cartesian(cuda, degree, refx, prec):
                                       ob.ctypes.c_double(0.3), int(refx))
     assert rel_err(to_np(d_new), r_new) < _tol(prec)
     assert rel_err(to_np(d_tm), r_tm) < _tol(prec)
+    # c_tm == 0 starts a recurrence: tmp is only written (NaN in it must not leak)
+    import torch
+    d_new, d_tm = to_dev(np.zeros(N), dt), torch.full((N,), float("nan"), dtype=dt, device="cuda")
+    op.cheby_step(to_dev(rhs, dt), to_dev(src, dt), to_dev(diag, dt), d_new, d_tm, 0.7, 0.0)
+    r_new, r_tm = np.zeros(N), np.zeros(N)
+    ob.oracle().sfko_dg_sipg_cheby_ex(degree, ob._i3(nc), ob._d3(ij), ob._ptr(rhs), ob._ptr(src), ob._ptr(diag),
+                                      ob._ptr(r_new), ob._ptr(r_tm), ob.ctypes.c_double(0.7),
+                                      ob.ctypes.c_double(0.0), int(refx))
+    assert rel_err(to_np(d_new), r_new) < _tol(prec)
+    assert rel_err(to_np(d_tm), r_tm) < _tol(prec)
 
 
 @pytest.mark.parametrize("prec", [sfk.FP64, sfk.FP32])

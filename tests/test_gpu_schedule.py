@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle_bindings as ob
-from helpers import TOL_FP64, rel_err, sine_metric, to_dev, to_np
+from helpers import TOL_FP32, TOL_FP64, rel_err, sine_metric, to_dev, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -32,7 +32,7 @@ def test_cont_schedules_match_oracle(cuda, degree):
     colored = {}
     for sched in _schedules(op, scatters=(0, 1)):
         op.set_schedule(*sched)
-        assert op.schedule == sched
+        assert op.schedule[:3] == sched
         out = to_np(op.apply(d_src))
         assert rel_err(out, ref) < TOL_FP64, sched
         if sched[2] == 1:
@@ -45,6 +45,92 @@ def test_cont_schedules_match_oracle(cuda, degree):
                 colored[cb] = out
             # repeated applies are deterministic
             assert np.array_equal(to_np(op.apply(d_src)), out)
+
+
+def _nan_vec(op, n=None):
+    import torch
+    return torch.full((op.vector_size * (n or 1),), float("nan"), dtype=op.dtype, device="cuda")
+
+
+@pytest.mark.parametrize("degree", range(1, 9))
+def test_cont_tp_schedules_match_oracle(cuda, degree):
+    """tensor-product kernel: every tile shape and z range (ragged tiles,
+    z ranges that start above the bottom, tiles larger than the mesh) against
+    the oracle's quadrature-form operator; every node is written (the output
+    starts as NaN), Dirichlet nodes exactly 0"""
+    nc = (7, 5, 6) if degree <= 4 else (5, 3, 4)
+    ij = (7.0, 2.5, 4.5)  # anisotropic cells: distinct direction constants
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    rng = np.random.default_rng(4050 + degree)
+    src = rng.uniform(-1, 1, op.vector_size)
+    ref = ob.cont_laplacian(degree, nc, ij, src)
+    d_src = to_dev(src)
+    g = [c * degree + 1 for c in nc]
+    count = op.tile_shape(0)[0]
+    for shape in range(count):
+        for zc in (0, 1, 2, 3, nc[2]):
+            for order in (0, 1):
+                op.set_schedule(0, order, 2, shape, zc)
+                out = to_np(op.apply(d_src, dst=_nan_vec(op)))
+                assert np.isfinite(out).all(), (shape, zc, order)
+                assert rel_err(out, ref) < TOL_FP64, (shape, zc, order)
+                o3 = out.reshape(g[2], g[1], g[0])
+                for face in (o3[0], o3[-1], o3[:, 0], o3[:, -1], o3[:, :, 0], o3[:, :, -1]):
+                    assert (face == 0).all(), (shape, zc, order)
+
+
+@pytest.mark.parametrize("degree,nc", [(4, (1, 1, 1)), (4, (2, 1, 3)), (6, (1, 2, 1)), (2, (40, 3, 2)),
+                                       (3, (3, 30, 2))])
+def test_cont_tp_small_and_long_meshes(cuda, degree, nc):
+    ij = tuple(float(c) for c in nc)
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    src = np.random.default_rng(4060 + degree).uniform(-1, 1, op.vector_size)
+    ref = ob.cont_laplacian(degree, nc, ij, src)
+    for shape in range(op.tile_shape(0)[0]):
+        op.set_schedule(0, 1, 2, shape, 0)
+        out = to_np(op.apply(to_dev(src), dst=_nan_vec(op)))
+        assert rel_err(out, ref) < TOL_FP64 or np.abs(ref).max() == 0 and (out == 0).all(), shape
+
+
+@pytest.mark.parametrize("degree", [2, 4, 6, 8])
+def test_cont_tp_fp32_multi_rhs(cuda, degree):
+    import torch
+    nc = (6, 5, 4) if degree <= 4 else (4, 3, 3)
+    ij = (6.0, 5.0, 4.0)
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij, precision=sfk.FP32, n_rhs=3)
+    op.set_schedule(0, 1, 2, 0, 2)
+    N = op.vector_size
+    rng = np.random.default_rng(4070 + degree)
+    src = rng.standard_normal(3 * N)
+    out = to_np(op.apply(to_dev(src, torch.float32), dst=_nan_vec(op, 3)))
+    for r in range(3):
+        ref = ob.cont_laplacian(degree, nc, ij, src[r * N:(r + 1) * N].copy())
+        assert rel_err(out[r * N:(r + 1) * N], ref) < TOL_FP32, r
+
+
+@pytest.mark.parametrize("degree", [1, 4, 6])
+@pytest.mark.parametrize("c_tm", [0.0, 0.37])
+@pytest.mark.parametrize("zc", [0, 1])
+def test_cont_tp_chebyshev_step(cuda, degree, c_tm, zc):
+    """Chebyshev update fused into the tensor-product kernel against
+    tm' = c_tm tm + c_np diag (A u - rhs), u' = u - tm' with A u from the
+    oracle, on every node (Dirichlet rows too); tm not read when c_tm == 0"""
+    nc = (5, 3, 4)
+    ij = (5.0, 3.0, 4.0)
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    op.set_schedule(0, 1, 2, 0, zc)
+    N = op.vector_size
+    rng = np.random.default_rng(4450 + degree)
+    u, rhs = rng.uniform(-1, 1, N), rng.standard_normal(N)
+    diag, tm = rng.uniform(0.5, 1.5, N), rng.standard_normal(N)
+    if c_tm == 0.0:
+        tm[:] = np.nan
+    d_new, d_tm = _nan_vec(op), to_dev(tm)
+    op.cheby_step(to_dev(rhs), to_dev(u), to_dev(diag), d_new, d_tm, 0.8, c_tm)
+    au = ob.cont_laplacian(degree, nc, ij, u)
+    r_tm = (c_tm * tm if c_tm != 0.0 else 0.0) + 0.8 * diag * (au - rhs)
+    assert rel_err(to_np(d_tm), r_tm) < TOL_FP64
+    assert rel_err(to_np(d_new), u - r_tm) < TOL_FP64
 
 
 @pytest.mark.parametrize("degree", [2, 5])
@@ -111,7 +197,7 @@ def test_tune_picks_valid_schedule(cuda):
     op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
     sched, t = op.tune(repeats=3)
     assert t > 0
-    assert 1 <= sched[0] <= op.max_cells_per_block and sched[1] in (0, 1) and sched[2] in (0, 1)
+    assert 1 <= sched[0] <= op.max_cells_per_block and sched[1] in (0, 1) and sched[2] in (0, 1, 2)
     assert op.schedule == sched
     src = np.random.default_rng(4300).uniform(-1, 1, op.vector_size)
     assert rel_err(to_np(op.apply(to_dev(src))), ob.cont_laplacian(degree, nc, ij, src)) < TOL_FP64
