@@ -42,6 +42,13 @@
 #include "sfk_device.h"
 #include "sfk_kernels.h"
 
+#ifndef SFK_TP_SCHED
+#define SFK_TP_SCHED 1
+#endif
+#ifndef SFK_TP_WAVES
+#define SFK_TP_WAVES 1
+#endif
+
 namespace sfk {
 
 // tables of the tensor-product operator (device memory, read through the
@@ -81,6 +88,37 @@ struct TpCfg {
   static constexpr int LDS_VALUES = 2 * LA + 2 * LC;
 };
 
+// waves per SIMD that the LDS of one tile shape allows (160 KiB per CU in
+// 1280-byte granules, at most 32 waves per CU): the kernel asks the
+// compiler for registers that let this many waves be resident, since an
+// allocation above 512/W VGPRs would drop a whole workgroup per CU
+template <int N, typename T, int BX, int BY>
+constexpr int tp_waves()
+{
+  using C = TpCfg<N, T, BX, BY>;
+  constexpr int bytes = C::LDS_VALUES * (int)sizeof(T);
+  constexpr int alloc = (bytes + 1279) / 1280 * 1280;
+  constexpr int wpg = C::THREADS / 64;
+  int wgs = 163840 / alloc;
+  if (wgs * wpg > 32) wgs = 32 / wpg;
+  const int w = (wgs * wpg + 3) / 4;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// element at a 32-bit byte offset (per lane) from a uniform base pointer:
+// the load or store takes the scalar base and the vector offset directly
+// (no per-lane 64-bit address arithmetic)
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, unsigned off)
+{
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ T &at(T *base, unsigned off)
+{
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+}
+
 template <typename T>
 struct TpArgs {
   const void *tab;
@@ -99,34 +137,38 @@ struct TpArgs {
 
 // element line with the shared first node: in[0..2p] = values at nodes
 // e p - p .. e p + p; out[i] = (A u_e)[i] + (i == 0 ? (A u_{e-1})[p] : 0),
-// i < p, for A = M (oa) and A = K (ob)
-template <int N, typename T, typename TB>
-__device__ __forceinline__ void line_mk(const TB &tb, int dir, const T (&in)[2 * N - 1], T (&oa)[N - 1],
-                                        T (&ob)[N - 1])
+// i < p, for A = M (W = 0) or c_d K (W = 1 + d).  The table pointer is
+// re-derived here and a scheduling barrier closes the contraction, so each
+// matrix lives in scalar registers only while it is used (held for a whole
+// phase, the tables of two matrices and their rows exceed the SGPR file).
+template <int N, typename T, int W>
+__device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 1], T (&o)[N - 1])
 {
   constexpr int p = N - 1;
-  T ue[N], ra[N], rb[N];
+  const auto &tb = tptab<N, T>(tab);
+  const auto &E = W == 0 ? tb.M : tb.K[W > 0 ? W - 1 : 0];
+  const auto &R = W == 0 ? tb.mrow : tb.krow[W > 0 ? W - 1 : 0];
+  T ue[N], r[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) ue[i] = in[p + i];
-  eo_apply<N, 1>(tb.M, ue, ra);
-  eo_apply<N, 1>(tb.K[dir], ue, rb);
-  T va = tb.mrow[0] * in[0], vb = tb.krow[dir][0] * in[0];
+  eo_apply<N, 1>(E, ue, r);
+  T v = R[0] * in[0];
 #pragma unroll
-  for (int j = 1; j < N; ++j) {
-    va = tb.mrow[j] * in[j] + va;
-    vb = tb.krow[dir][j] * in[j] + vb;
-  }
-  oa[0] = ra[0] + va;
-  ob[0] = rb[0] + vb;
+  for (int j = 1; j < N; ++j) v = R[j] * in[j] + v;
+  o[0] = r[0] + v;
 #pragma unroll
-  for (int i = 1; i < p; ++i) {
-    oa[i] = ra[i];
-    ob[i] = rb[i];
-  }
+  for (int i = 1; i < p; ++i) o[i] = r[i];
+#if SFK_TP_SCHED
+  __builtin_amdgcn_sched_barrier(0);
+#endif
 }
 
 template <int N, typename T, int BX, int BY, bool CHEBY>
-__global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel(const TpArgs<T> a)
+__global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS))
+#if SFK_TP_WAVES
+__attribute__((amdgpu_waves_per_eu(tp_waves<N, T, BX, BY>(), 8)))
+#endif
+cont_tp_kernel(const TpArgs<T> a)
 {
   using C = TpCfg<N, T, BX, BY>;
   constexpr int p = N - 1, TX = C::TX, TY = C::TY, HX = C::HX, RPA = C::RPA, RPC = C::RPC;
@@ -155,6 +197,10 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
   // y line (planes Z = ez p + k0 + s, s < NK, one work item per thread:
   // NK BY HX <= THREADS) into registers, store_y contracts it into slot s.
   static_assert(p * BY * HX <= C::THREADS, "one phase-Y item per thread");
+  // the y lines of the tile's cells ey0 .. ey0+BY-1 span the rows
+  // (ey0-1) p .. (ey0+BY) p: all interior node rows unless the tile is
+  // within one cell of the y boundary
+  const bool ybnd = !(ey0 >= 2 && ey0 + BY < a.ny);
   T u[2 * N - 1];
   bool yact = false;
   auto load_y = [&](int ez, int k0, auto nkc) {
@@ -166,11 +212,22 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
     yact = it < NK * BY * HX && ey < a.ny;
     const int X = x0 - p + xh, Z = ez * p + k0 + s;
     if (yact && X > 0 && X < gx - 1 && Z > 0 && Z < gz - 1) {
-      const T *col = src + (long long)Z * plane + X;
+      if (!ybnd) {
+        // interior tile row range: every row of the line is an interior
+        // node row; row j of all lanes is one uniform base + the lane's
+        // 32-bit offset within the layer (the host checks that the layer's
+        // planes span less than 2 GiB)
+        const T *lb = src + (long long)(ez * p) * plane;
+        const unsigned off = (unsigned)(((k0 + s) * (int)plane + (ey * p - p) * gx + X) * (int)sizeof(T));
 #pragma unroll
-      for (int j = 0; j < 2 * N - 1; ++j) {
-        const int Y = ey * p - p + j;
-        u[j] = (Y > 0 && Y < gy - 1) ? col[(long long)Y * gx] : T(0);
+        for (int j = 0; j < 2 * N - 1; ++j) u[j] = at(lb + j * gx, off);
+      } else {
+        const T *col = src + (long long)Z * plane + X;
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) {
+          const int Y = ey * p - p + j;
+          u[j] = (Y > 0 && Y < gy - 1) ? col[(long long)Y * gx] : T(0);
+        }
       }
     } else {
 #pragma unroll
@@ -179,11 +236,11 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
   };
   auto store_y = [&]() {
     if (!yact) return;
-    const auto &tb = tptab<N, T>(a.tab);
     const int it = threadIdx.x;
     const int xh = it % HX, rest = it / HX, eyl = rest % BY, s = rest / BY;
     T va[N - 1], vb[N - 1];
-    line_mk<N, T>(tb, 1, u, va, vb);
+    line_one<N, T, 0>(a.tab, u, va); // a = M_y u
+    line_one<N, T, 2>(a.tab, u, vb); // b = c_y K_y u
     T *ra = sA + (s * TY + eyl * p) * RPA + xh;
     T *rb = sB + (s * TY + eyl * p) * RPA + xh;
 #pragma unroll
@@ -196,30 +253,27 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
   // phase X: rows of slots s < NK
   auto phase_x = [&](auto nkc) {
     constexpr int NK = decltype(nkc)::value;
-    const auto &tb = tptab<N, T>(a.tab);
     for (int it = threadIdx.x; it < NK * TY * BX; it += C::THREADS) {
       const int yt = it % TY, rest = it / TY, s = rest % NK, exl = rest / NK;
       if (ex0 + exl >= a.nx || y0 + yt >= gy - 1) continue; // no owned output
       const T *ra = sA + (s * TY + yt) * RPA + exl * p;
       const T *rb = sB + (s * TY + yt) * RPA + exl * p;
-      T ua[2 * N - 1], ub[2 * N - 1];
-#pragma unroll
-      for (int j = 0; j < 2 * N - 1; ++j) {
-        ua[j] = ra[j];
-        ub[j] = rb[j];
-      }
-      // c = M a;  d = M b + c_x K a (K b, also formed by line_mk, is unused
-      // and dropped by the compiler)
-      T cm[N - 1], ck[N - 1], dm[N - 1], dk[N - 1];
-      line_mk<N, T>(tb, 0, ua, cm, ck);
-      line_mk<N, T>(tb, 0, ub, dm, dk);
       T *rc = sC + (s * TY + yt) * RPC + exl * p;
       T *rd = sD + (s * TY + yt) * RPC + exl * p;
+      // c = M a;  d = M b + c_x K a, one input line at a time (registers)
+      T ua[2 * N - 1], cm[N - 1], ck[N - 1];
 #pragma unroll
-      for (int i = 0; i < p; ++i) {
-        rc[i] = cm[i];
-        rd[i] = dm[i] + ck[i];
-      }
+      for (int j = 0; j < 2 * N - 1; ++j) ua[j] = ra[j];
+      line_one<N, T, 0>(a.tab, ua, cm);
+#pragma unroll
+      for (int i = 0; i < p; ++i) rc[i] = cm[i];
+      line_one<N, T, 1>(a.tab, ua, ck);
+      T ub[2 * N - 1], dm[N - 1];
+#pragma unroll
+      for (int j = 0; j < 2 * N - 1; ++j) ub[j] = rb[j];
+      line_one<N, T, 0>(a.tab, ub, dm);
+#pragma unroll
+      for (int i = 0; i < p; ++i) rd[i] = dm[i] + ck[i];
     }
   };
 
@@ -234,33 +288,38 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
   // written by the threads of the nodes before it
   const bool xtra_x = zone != 2 && Xt == TX - 1 && X + 1 == gx - 1;
   const bool xtra_y = zone != 2 && Yt == TY - 1 && Y + 1 == gy - 1;
-  const long long colg = (long long)Y * gx + X;
+  // byte offset of this thread's node column inside a plane (planes hold
+  // less than 2^31 bytes: checked on the host)
+  const unsigned coff = zone == 2 ? 0u : (unsigned)((Y * gx + X) * (int)sizeof(T));
   T cp = T(0), dp = T(0), yp = T(0);
 
-  auto put = [&](long long g, T v) {
+  // output of node (column offset off) in plane Z
+  auto put = [&](int Z, unsigned off, T v) {
+    const long long zo = (long long)Z * plane; // uniform
     if constexpr (CHEBY) {
-      // r = A u - rhs; tm = c_tm tm + c_np diag r; dst = u - tm
+      // r = A u - rhs; tm = c_tm tm + c_np diag r; dst = u - tm (five
+      // streams: lane addresses, not five uniform bases per plane, which
+      // would not fit the scalar registers)
+      const long long g = zo + off / (unsigned)sizeof(T);
       const T res = v - a.rhs[g];
       const T t = (a.c_tm != T(0) ? a.c_tm * a.tm[g] : T(0)) + a.c_np * a.diag[g] * res;
       a.tm[g] = t;
       dst[g] = src[g] - t;
     } else {
-      __builtin_nontemporal_store(v, dst + g);
+      __builtin_nontemporal_store(v, &at(dst + zo, off));
     }
   };
   // zero (Dirichlet) outputs of plane Z for this thread: its own node when
   // zone 1, the extra last-node columns
   auto put_zeros = [&](int Z) {
-    const long long g = (long long)Z * plane + colg;
-    if (zone == 1) put(g, T(0));
-    if (xtra_x) put(g + 1, T(0));
-    if (xtra_y) put(g + gx, T(0));
-    if (xtra_x && xtra_y) put(g + gx + 1, T(0));
+    if (zone == 1) put(Z, coff, T(0));
+    if (xtra_x) put(Z, coff + (unsigned)sizeof(T), T(0));
+    if (xtra_y) put(Z, coff + (unsigned)(gx * (int)sizeof(T)), T(0));
+    if (xtra_x && xtra_y) put(Z, coff + (unsigned)((gx + 1) * (int)sizeof(T)), T(0));
   };
 
   auto phase_z = [&](int ez, bool store) {
     if (zone == 0) {
-      const auto &tb = tptab<N, T>(a.tab);
       T c[N], d[N], o[N];
       c[0] = cp;
       d[0] = dp;
@@ -270,8 +329,11 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
         d[k] = sD[((k - 1) * TY + Yt) * RPC + Xt];
       }
       T om[N], ok[N];
-      eo_apply<N, 1>(tb.M, d, om);
-      eo_apply<N, 1>(tb.K[2], c, ok);
+      eo_apply<N, 1>(tptab<N, T>(a.tab).M, d, om);
+#if SFK_TP_SCHED
+      __builtin_amdgcn_sched_barrier(0);
+#endif
+      eo_apply<N, 1>(tptab<N, T>(a.tab).K[2], c, ok);
 #pragma unroll
       for (int k = 0; k < N; ++k) o[k] = om[k] + ok[k];
       o[0] += yp;
@@ -280,7 +342,7 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
 #pragma unroll
         for (int k = 0; k < p; ++k) {
           const int Z = ez * p + k;
-          put((long long)Z * plane + colg, (bxy || Z == 0) ? T(0) : o[k]);
+          put(Z, coff, (bxy || Z == 0) ? T(0) : o[k]);
         }
       }
       cp = c[p];
@@ -324,7 +386,7 @@ __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS)) cont_tp_kernel
   // the grid's last z plane (Dirichlet)
   if (ez_end == a.nz && zone != 2) {
     const int Z = gz - 1;
-    if (zone == 0) put((long long)Z * plane + colg, T(0));
+    if (zone == 0) put(Z, coff, T(0));
     put_zeros(Z);
   }
 }
@@ -432,6 +494,9 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   a.zc = L.zc > 0 ? (L.zc < a.nz ? L.zc : a.nz) : a.nz;
   a.order = L.order;
   a.vstride = L.vstride;
+  // 32-bit byte offsets: a plane, and the p+1 planes of a layer, hold less
+  // than 2^31 bytes
+  if ((long long)N * a.gx * a.gy * (long long)sizeof(T) >= (1LL << 31)) return hipErrorInvalidValue;
   const long long nchunks = (a.nz + a.zc - 1) / a.zc;
   const long long blocks = (long long)a.ntx * a.nty * nchunks;
   if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
@@ -445,16 +510,16 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
 
 // tile shapes (BX, BY) per points per direction N; shape 0 is the default
 #define SFK_TP_SHAPES(X)                                                                       \
-  X(2, 0, 32, 16) X(2, 1, 64, 8) X(2, 2, 16, 32)                                               \
-  X(3, 0, 16, 8) X(3, 1, 32, 4) X(3, 2, 8, 16)                                                 \
-  X(4, 0, 8, 8) X(4, 1, 16, 4) X(4, 2, 4, 16)                                                  \
-  X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 4, 8)                                                   \
-  X(6, 0, 4, 4) X(6, 1, 8, 2) X(6, 2, 8, 4)                                                    \
-  X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2)                                                    \
-  X(8, 0, 4, 2) X(8, 1, 2, 4) X(8, 2, 2, 2)                                                    \
-  X(9, 0, 4, 2) X(9, 1, 2, 2) X(9, 2, 4, 1)
+  X(2, 0, 32, 16) X(2, 1, 64, 8) X(2, 2, 16, 32) X(2, 3, 32, 8)                                 \
+  X(3, 0, 16, 8) X(3, 1, 32, 4) X(3, 2, 8, 16) X(3, 3, 16, 4)                                   \
+  X(4, 0, 8, 8) X(4, 1, 16, 4) X(4, 2, 4, 16) X(4, 3, 8, 4)                                     \
+  X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 8, 2) X(5, 3, 16, 1)                                     \
+  X(6, 0, 4, 4) X(6, 1, 8, 2) X(6, 2, 8, 4) X(6, 3, 4, 2)                                       \
+  X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2) X(7, 3, 8, 1)                                       \
+  X(8, 0, 4, 2) X(8, 1, 2, 4) X(8, 2, 2, 2) X(8, 3, 4, 1)                                       \
+  X(9, 0, 4, 2) X(9, 1, 2, 2) X(9, 2, 4, 1) X(9, 3, 2, 1)
 
-constexpr int kTpShapes = 3;
+constexpr int kTpShapes = 4;
 
 int tp_shape_count(int) { return kTpShapes; }
 
