@@ -543,7 +543,7 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
       // the update fused into the tensor-product kernel's stores
       TpLaunch T;
       fill_tp_launch(op, T);
-      T.cheby = true;
+      T.cheby = 1;
       T.src = sol_old;
       T.dst = sol_new;
       T.rhs = rhs;

@@ -59,6 +59,11 @@ struct TpTab {
   Eo<N, T> K[3]; // c_d K, d = x, y, z
   T mrow[N];     // last row of M
   T krow[3][N];  // last row of c_d K
+  // diagonals of the assembled 1D matrices at local node index i (i = 0: a
+  // vertex shared by two cells): the operator's diagonal at node (i, j, k)
+  // is dk[0][i] dm[j] dm[k] + dm[i] dk[1][j] dm[k] + dm[i] dm[j] dk[2][k]
+  T dm[N];
+  T dk[3][N];
 };
 
 template <int N, typename T>
@@ -126,6 +131,7 @@ struct TpArgs {
   T *dst;
   const T *rhs, *diag; // fused Chebyshev
   T *tm;
+  const T *prev;       // three-term Chebyshev: the previous iterate (or null: zero)
   T c_np, c_tm;
   int nx, ny, nz;    // cells
   int gx, gy, gz;    // nodes
@@ -141,6 +147,14 @@ struct TpArgs {
 // re-derived here and a scheduling barrier closes the contraction, so each
 // matrix lives in scalar registers only while it is used (held for a whole
 // phase, the tables of two matrices and their rows exceed the SGPR file).
+//
+// The stiffness rows sum to zero (K 1 = 0), so K u = K (u - s) for any
+// constant s: the stiffness contractions take the differences to the line's
+// shared node (s = in[p], exact for the neighbouring values of a smooth u)
+// instead of the values themselves.  For smooth u the sum then no longer
+// cancels terms of the size of u, which in FP32 made A u of the low modes
+// inaccurate (relative error ~1e-7 times the condition number) and cost the
+// mixed-precision solver iterations.
 template <int N, typename T, int W>
 __device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 1], T (&o)[N - 1])
 {
@@ -148,13 +162,14 @@ __device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 
   const auto &tb = tptab<N, T>(tab);
   const auto &E = W == 0 ? tb.M : tb.K[W > 0 ? W - 1 : 0];
   const auto &R = W == 0 ? tb.mrow : tb.krow[W > 0 ? W - 1 : 0];
+  const T sh = W == 0 ? T(0) : in[p];
   T ue[N], r[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) ue[i] = in[p + i];
+  for (int i = 0; i < N; ++i) ue[i] = W == 0 ? in[p + i] : in[p + i] - sh;
   eo_apply<N, 1>(E, ue, r);
-  T v = R[0] * in[0];
+  T v = R[0] * (W == 0 ? in[0] : in[0] - sh);
 #pragma unroll
-  for (int j = 1; j < N; ++j) v = R[j] * in[j] + v;
+  for (int j = 1; j < N; ++j) v = R[j] * (W == 0 ? in[j] : in[j] - sh) + v;
   o[0] = r[0] + v;
 #pragma unroll
   for (int i = 1; i < p; ++i) o[i] = r[i];
@@ -163,7 +178,14 @@ __device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 
 #endif
 }
 
-template <int N, typename T, int BX, int BY, bool CHEBY>
+// CHEBY: 0 matvec; 1 the Chebyshev step of do_cheby_iter<true>
+// (evaluation_dg_laplacian.h:485-502): tm = c_tm tm + c_np diag (A u - rhs),
+// dst = u - tm; 2 the same recurrence in three-term form with the
+// operator's own inverse diagonal (solver smoother): with tm = prev - u,
+// dst = u - c_tm (prev - u) - c_np D^{-1} (A u - rhs), four streams instead
+// of six, and dst may be prev (each node is read, then written, by one
+// thread)
+template <int N, typename T, int BX, int BY, int CHEBY>
 __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS))
 #if SFK_TP_WAVES
 __attribute__((amdgpu_waves_per_eu(tp_waves<N, T, BX, BY>(), 8)))
@@ -293,10 +315,16 @@ cont_tp_kernel(const TpArgs<T> a)
   const unsigned coff = zone == 2 ? 0u : (unsigned)((Y * gx + X) * (int)sizeof(T));
   T cp = T(0), dp = T(0), yp = T(0);
 
-  // output of node (column offset off) in plane Z
+  // output of node (column offset off) in plane Z (Dirichlet nodes only for
+  // CHEBY == 2: the inverse diagonal is 0 there)
   auto put = [&](int Z, unsigned off, T v) {
     const long long zo = (long long)Z * plane; // uniform
-    if constexpr (CHEBY) {
+    if constexpr (CHEBY == 2) {
+      const long long g = zo + off / (unsigned)sizeof(T);
+      const T us = src[g];
+      const T pv = a.prev ? a.prev[g] : T(0); // null: the zero vector
+      dst[g] = us - a.c_tm * (pv - us);
+    } else if constexpr (CHEBY == 1) {
       // r = A u - rhs; tm = c_tm tm + c_np diag r; dst = u - tm (five
       // streams: lane addresses, not five uniform bases per plane, which
       // would not fit the scalar registers)
@@ -309,10 +337,70 @@ cont_tp_kernel(const TpArgs<T> a)
       __builtin_nontemporal_store(v, &at(dst + zo, off));
     }
   };
+  // fused Chebyshev: rhs, diag, tmp and the input at this thread's node
+  // column for the p planes of a layer, loaded when the layer starts and in
+  // flight during phases Y and X (loaded when phase Z needs them, their
+  // latency was not hidden: a step took 4x a matvec)
+  constexpr int NCH = CHEBY ? p : 1;
+  T q_r[NCH], q_d[CHEBY == 1 ? NCH : 1], q_t[NCH], q_s[NCH];
+  auto load_cheby = [&](int ez) {
+    if constexpr (CHEBY != 0) {
+      if (zone != 2) {
+        const long long c0 = coff / (unsigned)sizeof(T);
+#pragma unroll
+        for (int k = 0; k < p; ++k) {
+          const long long g = (long long)(ez * p + k) * plane + c0;
+          q_r[k] = a.rhs[g];
+          if constexpr (CHEBY == 1) {
+            q_d[k] = a.diag[g];
+            q_t[k] = a.c_tm != T(0) ? a.tm[g] : T(0);
+          } else {
+            q_t[k] = a.prev ? a.prev[g] : T(0);
+          }
+          q_s[k] = src[g];
+        }
+      }
+    }
+  };
+  // CHEBY == 2: the x/y part of this node column's diagonal, D = d1 dm[k] +
+  // d2 dk[2][k] in plane k of a cell (0 off the interior columns)
+  T d1 = T(0), d2 = T(0);
+  if constexpr (CHEBY == 2) {
+    if (zone == 0 && X > 0 && Y > 0) {
+      const auto &tb = tptab<N, T>(a.tab);
+      const int ix = X % p, iy = Y % p;
+      d1 = tb.dk[0][ix] * tb.dm[iy] + tb.dm[ix] * tb.dk[1][iy];
+      d2 = tb.dm[ix] * tb.dm[iy];
+    }
+  }
+  // output v of this thread's node in plane ez p + k (prefetched streams)
+  auto put_own = [&](int ez, int k, T v) {
+    if constexpr (CHEBY == 2) {
+      const long long g = (long long)(ez * p + k) * plane + coff / (unsigned)sizeof(T);
+      T dinv = T(0);
+      if (d2 != T(0) && ez * p + k > 0) {
+        const auto &tb = tptab<N, T>(a.tab);
+        dinv = T(1) / (d1 * tb.dm[k] + d2 * tb.dk[2][k]);
+      }
+      const T t = a.c_tm * (q_t[k] - q_s[k]) + a.c_np * dinv * (v - q_r[k]);
+      dst[g] = q_s[k] - t;
+    } else if constexpr (CHEBY == 1) {
+      const long long g = (long long)(ez * p + k) * plane + coff / (unsigned)sizeof(T);
+      const T res = v - q_r[k];
+      const T t = (a.c_tm != T(0) ? a.c_tm * q_t[k] : T(0)) + a.c_np * q_d[k] * res;
+      a.tm[g] = t;
+      dst[g] = q_s[k] - t;
+    } else {
+      put(ez * p + k, coff, v);
+    }
+  };
   // zero (Dirichlet) outputs of plane Z for this thread: its own node when
-  // zone 1, the extra last-node columns
-  auto put_zeros = [&](int Z) {
-    if (zone == 1) put(Z, coff, T(0));
+  // zone 1 (plane ez p + k), the extra last-node columns
+  auto put_zeros = [&](int Z, int ez, int k) {
+    if (zone == 1) {
+      if (k >= 0) put_own(ez, k, T(0));
+      else put(Z, coff, T(0));
+    }
     if (xtra_x) put(Z, coff + (unsigned)sizeof(T), T(0));
     if (xtra_y) put(Z, coff + (unsigned)(gx * (int)sizeof(T)), T(0));
     if (xtra_x && xtra_y) put(Z, coff + (unsigned)((gx + 1) * (int)sizeof(T)), T(0));
@@ -333,7 +421,13 @@ cont_tp_kernel(const TpArgs<T> a)
 #if SFK_TP_SCHED
       __builtin_amdgcn_sched_barrier(0);
 #endif
-      eo_apply<N, 1>(tptab<N, T>(a.tab).K[2], c, ok);
+      {
+        // differences to the line's first node (K 1 = 0, see line_one)
+        T cs[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) cs[k] = c[k] - c[0];
+        eo_apply<N, 1>(tptab<N, T>(a.tab).K[2], cs, ok);
+      }
 #pragma unroll
       for (int k = 0; k < N; ++k) o[k] = om[k] + ok[k];
       o[0] += yp;
@@ -342,7 +436,7 @@ cont_tp_kernel(const TpArgs<T> a)
 #pragma unroll
         for (int k = 0; k < p; ++k) {
           const int Z = ez * p + k;
-          put(Z, coff, (bxy || Z == 0) ? T(0) : o[k]);
+          put_own(ez, k, (bxy || Z == 0) ? T(0) : o[k]);
         }
       }
       cp = c[p];
@@ -350,7 +444,8 @@ cont_tp_kernel(const TpArgs<T> a)
       yp = o[p];
     }
     if (store && zone != 2)
-      for (int k = 0; k < p; ++k) put_zeros(ez * p + k);
+#pragma unroll
+      for (int k = 0; k < p; ++k) put_zeros(ez * p + k, ez, k);
   };
 
   using one = std::integral_constant<int, 1>;
@@ -358,6 +453,7 @@ cont_tp_kernel(const TpArgs<T> a)
   // one layer: phase Y from the prefetched registers, the next layer's
   // loads, phases X and Z
   auto layer = [&](int ez, bool store, int ez_next) {
+    if (store) load_cheby(ez);
     store_y();
     __syncthreads();
     if (ez_next >= 0) load_y(ez_next, 1, all());
@@ -387,7 +483,7 @@ cont_tp_kernel(const TpArgs<T> a)
   if (ez_end == a.nz && zone != 2) {
     const int Z = gz - 1;
     if (zone == 0) put(Z, coff, T(0));
-    put_zeros(Z);
+    put_zeros(Z, 0, -1);
   }
 }
 
@@ -438,6 +534,13 @@ static TpTab<N, T> make_tptab(const Tables1D &t, const double inv_jac[3])
     for (int j = 0; j < N; ++j) tb.krow[d][j] = T(Kd[(N - 1) * N + j]);
   }
   for (int j = 0; j < N; ++j) tb.mrow[j] = T(M[(N - 1) * N + j]);
+  for (int i = 0; i < N; ++i) {
+    // node i of a cell; i = 0 is also node p of the cell before it
+    const double m = i == 0 ? M[0] + M[N * N - 1] : M[i * N + i];
+    const double k = i == 0 ? K[0] + K[N * N - 1] : K[i * N + i];
+    tb.dm[i] = T(m);
+    for (int d = 0; d < 3; ++d) tb.dk[d][i] = T(inv_jac[d] * inv_jac[d] * jxw * k);
+  }
   return tb;
 }
 
@@ -481,6 +584,7 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   a.rhs = (const T *)L.rhs;
   a.diag = (const T *)L.diag;
   a.tm = (T *)L.tm;
+  a.prev = (const T *)L.prev;
   a.c_np = T(L.c_np);
   a.c_tm = T(L.c_tm);
   a.nx = L.nc[0];
@@ -501,7 +605,8 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   const long long blocks = (long long)a.ntx * a.nty * nchunks;
   if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
   const size_t lds = size_t(C::LDS_VALUES) * sizeof(T);
-  auto kern = L.cheby ? cont_tp_kernel<N, T, BX, BY, true> : cont_tp_kernel<N, T, BX, BY, false>;
+  auto kern = L.cheby == 2 ? cont_tp_kernel<N, T, BX, BY, 2>
+                            : (L.cheby == 1 ? cont_tp_kernel<N, T, BX, BY, 1> : cont_tp_kernel<N, T, BX, BY, 0>);
   hipError_t e = allow_dynamic_lds((const void *)kern, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks, L.nrhs), dim3(C::THREADS), lds, s, a);

@@ -50,12 +50,14 @@ hipError_t launch_cell(const CellLaunch &L, hipStream_t stream);
 struct TpLaunch {
   int n = 0;            // points per direction (degree + 1)
   bool fp32 = false;
-  bool cheby = false;   // fused Chebyshev update
+  int cheby = 0;        // 0 matvec, 1 fused Chebyshev step (tm form), 2 three-term
+                        // Chebyshev step with the operator's inverse diagonal
   const void *tptab = nullptr; // TpTab<n, T> in device memory
   const void *src = nullptr;
   void *dst = nullptr;
   const void *rhs = nullptr, *diag = nullptr;
   void *tm = nullptr;
+  const void *prev = nullptr; // cheby 2: previous iterate (null: zero)
   double c_np = 0., c_tm = 0.;
   int nc[3] = {1, 1, 1};
   int nrhs = 1;
@@ -176,6 +178,8 @@ hipError_t vec_cg_update(long long n, double alpha, const void *p, const void *q
 hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, const void *q, double *result_dev,
                           double *partials, int n_partials, hipStream_t s);
 hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, void *y, hipStream_t s);
+// z = c a b (elementwise)
+hipError_t vec_mul_scaled(bool fp32, long long n, double c, const void *a, const void *b, void *z, hipStream_t s);
 // p = (double) zs + b p (zs FP32, p FP64)
 hipError_t vec_axpby_widen(long long n, const void *zs, double b, void *p, hipStream_t s);
 hipError_t vec_cheby_update(bool fp32, long long n, const void *au, const void *rhs, const void *sol_old,

@@ -79,5 +79,10 @@ inline void fill_precomp_launch(const sfk_op *op, PrecompLaunch &L)
 }
 
 size_t elem_size(const sfk_op *op);
+// device tables of the operator (uploaded on first use)
+sfk_status ensure_device_tables(sfk_op *op);
+// the continuous Laplacian's launches use the tensor-product kernel
+bool uses_tp(const sfk_op *op);
+void fill_tp_launch(const sfk_op *op, TpLaunch &L);
 
 } // namespace sfk

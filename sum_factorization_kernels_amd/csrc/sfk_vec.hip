@@ -172,6 +172,14 @@ __global__ void cheby_kernel(long long n, const T *au, const T *rhs, const T *so
   }
 }
 
+// z = c a b (elementwise)
+template <typename T>
+__global__ void mul_scaled_kernel(long long n, T c, const T *a, const T *b, T *z)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    z[i] = c * a[i] * b[i];
+}
+
 template <typename T>
 __global__ void dirichlet_kernel(T *x, int gx, int gy, int gz)
 {
@@ -296,6 +304,17 @@ hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, 
     hipLaunchKernelGGL(axpby_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, (float)a, (const float *)x, (float)b, (float *)y);
   else
     hipLaunchKernelGGL(axpby_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, a, (const double *)x, b, (double *)y);
+  return hipGetLastError();
+}
+
+hipError_t vec_mul_scaled(bool fp32, long long n, double c, const void *a, const void *b, void *z, hipStream_t s)
+{
+  if (fp32)
+    hipLaunchKernelGGL(mul_scaled_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, (float)c,
+                       (const float *)a, (const float *)b, (float *)z);
+  else
+    hipLaunchKernelGGL(mul_scaled_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, c,
+                       (const double *)a, (const double *)b, (double *)z);
   return hipGetLastError();
 }
 
