@@ -147,14 +147,6 @@ struct TpArgs {
 // re-derived here and a scheduling barrier closes the contraction, so each
 // matrix lives in scalar registers only while it is used (held for a whole
 // phase, the tables of two matrices and their rows exceed the SGPR file).
-//
-// The stiffness rows sum to zero (K 1 = 0), so K u = K (u - s) for any
-// constant s: the stiffness contractions take the differences to the line's
-// shared node (s = in[p], exact for the neighbouring values of a smooth u)
-// instead of the values themselves.  For smooth u the sum then no longer
-// cancels terms of the size of u, which in FP32 made A u of the low modes
-// inaccurate (relative error ~1e-7 times the condition number) and cost the
-// mixed-precision solver iterations.
 template <int N, typename T, int W>
 __device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 1], T (&o)[N - 1])
 {
@@ -162,14 +154,13 @@ __device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 
   const auto &tb = tptab<N, T>(tab);
   const auto &E = W == 0 ? tb.M : tb.K[W > 0 ? W - 1 : 0];
   const auto &R = W == 0 ? tb.mrow : tb.krow[W > 0 ? W - 1 : 0];
-  const T sh = W == 0 ? T(0) : in[p];
   T ue[N], r[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) ue[i] = W == 0 ? in[p + i] : in[p + i] - sh;
+  for (int i = 0; i < N; ++i) ue[i] = in[p + i];
   eo_apply<N, 1>(E, ue, r);
-  T v = R[0] * (W == 0 ? in[0] : in[0] - sh);
+  T v = R[0] * in[0];
 #pragma unroll
-  for (int j = 1; j < N; ++j) v = R[j] * (W == 0 ? in[j] : in[j] - sh) + v;
+  for (int j = 1; j < N; ++j) v = R[j] * in[j] + v;
   o[0] = r[0] + v;
 #pragma unroll
   for (int i = 1; i < p; ++i) o[i] = r[i];
@@ -421,13 +412,7 @@ cont_tp_kernel(const TpArgs<T> a)
 #if SFK_TP_SCHED
       __builtin_amdgcn_sched_barrier(0);
 #endif
-      {
-        // differences to the line's first node (K 1 = 0, see line_one)
-        T cs[N];
-#pragma unroll
-        for (int k = 0; k < N; ++k) cs[k] = c[k] - c[0];
-        eo_apply<N, 1>(tptab<N, T>(a.tab).K[2], cs, ok);
-      }
+      eo_apply<N, 1>(tptab<N, T>(a.tab).K[2], c, ok);
 #pragma unroll
       for (int k = 0; k < N; ++k) o[k] = om[k] + ok[k];
       o[0] += yp;
