@@ -300,6 +300,17 @@ sfk_status sfk_synchronize(void *stream);
 sfk_status sfk_vec_fill(int precision, void *x, long long n, double value, void *stream);
 sfk_status sfk_vec_dot(int precision, const void *x, const void *y, long long n, double *result,
                        void *stream);
+/* Allocation-free form of sfk_vec_dot.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  *result_dev is
+ * device memory and receives x.y (FP64-accumulated), stream-ordered, with no
+ * synchronisation.  The partial sums live in a static device array of 4096
+ * doubles per device (one per chunk of 65536 elements), so n is limited to
+ * 4096 * 65536 = 268435456 elements (larger n: SFK_ERR_INVALID_ARG, nothing
+ * launched).  Calls that overlap on one device race on that array; the
+ * result depends on n only (fixed chunks), not on the launch. */
+sfk_status sfk_vec_dot_noalloc(int precision, const void *x, const void *y, long long n, double *result_dev,
+                               void *stream);
 /* unfused Chebyshev vector update (EvaluationDGLaplacian::
  * emulate_cheby_vector_updates, evaluation_dg_laplacian.h:556-601, with
  * au = A sol_old supplied by the caller):
