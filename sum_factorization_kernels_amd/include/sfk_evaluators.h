@@ -6,6 +6,7 @@
 //   EvaluationDGLaplacian<dim,degree,Number>    legacy/include/evaluation_dg_laplacian.h:33-716
 //   EvaluationDGAdvection<dim,degree,Number>    legacy/include/evaluation_dg_advect.h:96-845
 //   PrecomputedDGLaplacian<dim,degree,Number>   legacy/include/precomputed_dg_laplacian.h:33-389
+//   EvaluationContinuousLaplacian<dim,degree,Number>  (no reference counterpart)
 //
 // As in the reference, an evaluator owns its vectors (here in device memory,
 // canonical cell-major layout) and initialize() sets the initial data.  Only
@@ -101,6 +102,19 @@ public:
     long long n = 0;
     check(sfk_op_vector_size(op_, &n));
     return n;
+  }
+  // quadrature points of all cells, and face points per direction
+  long long n_cell_q() const
+  {
+    long long c = 0, f = 0;
+    check(sfk_op_qdata_sizes(op_, &c, &f));
+    return c;
+  }
+  long long n_face_q() const
+  {
+    long long c = 0, f = 0;
+    check(sfk_op_qdata_sizes(op_, &c, &f));
+    return f;
   }
 
 private:
@@ -340,6 +354,12 @@ public:
   static constexpr unsigned int dofs_per_cell = n_q_points;
   unsigned int blx = 1, bly = 1, blz = 1;
 
+  // Per-point velocity (no reference counterpart): set before initialize()
+  // to keep the velocity per cell and face quadrature point (initialized to
+  // the constant field a; set_velocity replaces it), which the gradient
+  // grad_velocity needs.
+  bool variable_velocity = false;
+
   // domain [0,2]^3 (evaluation_dg_advect.h:719-727); sol_new = exact
   // solution at the GLL nodes, time step 0.2 h / |a| / p (:160-165)
   void initialize(const unsigned int *n_cells_in)
@@ -353,7 +373,20 @@ public:
     sfk_op_desc d = sfk::make_desc(SFK_DG_ADVECT, degree, sfk::precision_of<Number>(), SFK_BASIS_GLL, n_cells, inv_jac);
     ExactSolution<dim> ex;
     for (int k = 0; k < 3; ++k) d.velocity[k] = ex.a[k];
+    if (variable_velocity) d.flags |= SFK_FLAG_VARIABLE_VELOCITY;
     op_.reset(new sfk::Operator(d));
+    if (variable_velocity) {
+      // the constant field at every point: cell [cell][3][q] = a, face
+      // [d][cell][q] = a . e_d (the upper face's unit normal)
+      const std::size_t nq = dofs_per_cell, nf = (degree + 1) * (degree + 1), nc = n_elements();
+      std::vector<double> cell(nc * 3 * nq), face(3 * nc * nf);
+      for (std::size_t c = 0; c < nc; ++c)
+        for (int k = 0; k < 3; ++k)
+          for (std::size_t q = 0; q < nq; ++q) cell[(c * 3 + k) * nq + q] = ex.a[k];
+      for (int k = 0; k < 3; ++k)
+        for (std::size_t i = 0; i < nc * nf; ++i) face[k * nc * nf + i] = ex.a[k];
+      set_velocity(cell, face);
+    }
     const std::size_t n = n_elements() * dofs_per_cell;
     for (auto *v : {&sol_old, &sol_new, &sol_tmp}) v->resize(n);
     sol_new.upload(nodal(ex, false));
@@ -421,6 +454,32 @@ public:
     } else {
       sfk::check(sfk_apply(op_->get(), src.data(), vec_np.data(), nullptr));
     }
+    sfk::check(sfk_synchronize(nullptr));
+  }
+
+  // ---- adjoint / gradient API (no reference counterpart; the forward
+  // operator is do_matvec's M^{-1} L, evaluation_dg_advect.h:285-676)
+  // velocity per point, layouts of sfk_op_set_velocity: cell [cell][3][q],
+  // face [d][cell][q] (a . n on the upper face in direction d)
+  void set_velocity(const std::vector<double> &cell, const std::vector<double> &face)
+  {
+    sfk::check(sfk_op_set_velocity(op_->get(), cell.data(), cell.size(), face.data(), face.size()));
+  }
+  std::size_t n_cell_q() const { return (std::size_t)op_->n_cell_q(); }
+  std::size_t n_face_q() const { return (std::size_t)op_->n_face_q(); }
+  // dst = A^T src (the downwind transpose of M^{-1} L)
+  void apply_transpose(const sfk::DeviceVector<Number> &src, sfk::DeviceVector<Number> &dst)
+  {
+    sfk::check(sfk_apply_transpose(op_->get(), src.data(), dst.data(), nullptr));
+    sfk::check(sfk_synchronize(nullptr));
+  }
+  // gradient of lambda^T A(a) u with respect to the per-point velocity a
+  // (with lambda = A u - f: the gradient of 0.5 |A u - f|^2); grad_cell has
+  // 3 n_cell_q() entries, grad_face 3 n_face_q() (variable_velocity only)
+  void grad_velocity(const sfk::DeviceVector<Number> &u, const sfk::DeviceVector<Number> &lambda,
+                     sfk::DeviceVector<Number> &grad_cell, sfk::DeviceVector<Number> &grad_face)
+  {
+    sfk::check(sfk_grad_velocity(op_->get(), u.data(), lambda.data(), grad_cell.data(), grad_face.data(), nullptr));
     sfk::check(sfk_synchronize(nullptr));
   }
 
@@ -492,6 +551,78 @@ private:
   double graph_dt_ = 0.;
 };
 
+// --------------------------------------------------------------------------
+// Continuous Q_p Laplacian -div(k grad u) with homogeneous Dirichlet rows on
+// the uniform mesh of [0,1]^3 (no reference counterpart: the reference's
+// cell Laplacian is element-local, evaluation_cell_laplacian.h:221-222).
+// Vectors hold the (nx p+1)(ny p+1)(nz p+1) nodes, x fastest, n_rhs of them
+// back to back; the coefficient k, when variable, is given per cell
+// quadrature point.  Adjoint API: the operator is symmetric
+// (apply_transpose == matrix_vector_product), and grad_coefficient returns
+// d(lambda^T A(k) u)/dk per quadrature point.
+template <int dim, int degree, typename Number>
+class EvaluationContinuousLaplacian {
+  static_assert(dim == 3, "three-dimensional operators only");
+
+public:
+  void initialize(const unsigned int *n_cells_in, unsigned n_rhs_in = 1, bool variable_coefficient = false)
+  {
+    double ij[3];
+    for (int d = 0; d < 3; ++d) {
+      n_cells[d] = n_cells_in[d];
+      ij[d] = n_cells[d];
+    }
+    sfk_op_desc d = sfk::make_desc(SFK_CONT_LAPLACE, degree, sfk::precision_of<Number>(), SFK_BASIS_GLL, n_cells, ij);
+    d.n_rhs = (int)n_rhs_in;
+    if (variable_coefficient) d.flags |= SFK_FLAG_VARIABLE_COEF;
+    op_.reset(new sfk::Operator(d));
+    n_rhs = n_rhs_in;
+    input_array.resize(vector_size() * n_rhs);
+    output_array.resize(vector_size() * n_rhs);
+    input_array.fill(0.);
+    output_array.fill(0.);
+  }
+  std::size_t vector_size() const { return (std::size_t)op_->vector_size(); }
+  std::size_t n_cell_q() const { return (std::size_t)op_->n_cell_q(); }
+  void set_coefficient(const std::vector<double> &k)
+  {
+    sfk::check(sfk_op_set_coefficient(op_->get(), k.data(), k.size()));
+  }
+  // output_array = A input_array (all right-hand sides)
+  void matrix_vector_product()
+  {
+    sfk::check(sfk_apply(op_->get(), input_array.data(), output_array.data(), nullptr));
+    sfk::check(sfk_synchronize(nullptr));
+  }
+  void apply_transpose(const sfk::DeviceVector<Number> &src, sfk::DeviceVector<Number> &dst)
+  {
+    sfk::check(sfk_apply_transpose(op_->get(), src.data(), dst.data(), nullptr));
+    sfk::check(sfk_synchronize(nullptr));
+  }
+  // grad[q] = sum over the right-hand sides of JxW_q grad u_q . grad lambda_q
+  void grad_coefficient(const sfk::DeviceVector<Number> &u, const sfk::DeviceVector<Number> &lambda,
+                        sfk::DeviceVector<Number> &grad)
+  {
+    sfk::check(sfk_grad_coefficient(op_->get(), u.data(), lambda.data(), grad.data(), nullptr));
+    sfk::check(sfk_synchronize(nullptr));
+  }
+  // the autotuner's schedule for matrix_vector_product (seconds per apply)
+  double tune(int repeats = 5)
+  {
+    double t = 0.;
+    sfk::check(sfk_tune(op_->get(), repeats, nullptr, &t, nullptr));
+    return t;
+  }
+  sfk_op *handle() const { return op_->get(); }
+
+  unsigned int n_cells[3] = {1, 1, 1};
+  unsigned n_rhs = 1;
+  sfk::DeviceVector<Number> input_array, output_array;
+
+private:
+  std::unique_ptr<sfk::Operator> op_;
+};
+
 // ------------------------------------------------------------------ bodies
 template <int dim, int degree, typename Number>
 double EvaluationCellLaplacian<dim, degree, Number>::do_verification()
@@ -527,7 +658,7 @@ double EvaluationCellLaplacian<dim, degree, Number>::do_verification()
   // Hermite-like basis the coefficients of the linear function x
   std::vector<double> lin(n);
   if (basis == SFK_BASIS_GLL) {
-    lin = gl;
+    for (int i = 0; i < n; ++i) lin[i] = gl[i];
   } else {
     // x = sum_i c_i phi_i(x): phi_0 + phi_1 carry the value and slope at 0
     // (phi_1'(0) = -phi_0'(0)), solve on the Gauss points

@@ -97,7 +97,7 @@ def test_advect_objective_gradient_48cube(cuda):
 
 @pytest.mark.parametrize("prec", [sfk.FP64, sfk.FP32])
 def test_advect_48cube_matches_oracle(cuda, prec):
-    """forward, adjoint and (FP64) velocity gradient at 48^3 cells, degree 3,
+    """forward, adjoint and velocity gradient at 48^3 cells, degree 3,
     against the FP64 CPU oracle on the same inputs (seed 4)"""
     import torch
     degree, nc = 3, (48, 48, 48)
@@ -113,8 +113,9 @@ def test_advect_48cube_matches_oracle(cuda, prec):
     assert rel_err(to_np(op.apply(to_dev(u, dt))), au) < tol
     lam = au - f
     assert rel_err(to_np(op.apply_transpose(to_dev(lam, dt))), ob.dg_advect_var_transpose(degree, nc, ij, cv, fv, lam)) < tol
-    if prec == sfk.FP64:
-        gc, gf = op.grad_velocity(to_dev(u), to_dev(lam))
-        rc, rf = ob.dg_advect_velocity_grad(degree, nc, ij, fv, u, lam)
-        assert rel_err(to_np(gc), rc) < TOL_FP64
-        assert rel_err(to_np(gf), rf) < TOL_FP64
+    # velocity gradient in the operator's precision against the FP64 oracle
+    # (relative to the largest entry: 1e-12 in FP64, 2e-5 in FP32)
+    gc, gf = op.grad_velocity(to_dev(u, dt), to_dev(lam, dt))
+    rc, rf = ob.dg_advect_velocity_grad(degree, nc, ij, fv, u, lam)
+    assert rel_err(to_np(gc), rc) < tol
+    assert rel_err(to_np(gf), rf) < tol
