@@ -1,7 +1,7 @@
 """Benchmark of the MI355X sum-factorization operators.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
-                    [--only WORKLOAD] [--copy-probe]
+                    [--only WORKLOAD [--schedule CB,ORDER,SCATTER,SHAPE,ZC]] [--copy-probe]
 
 Headline workload (one JSON line on stdout): the continuous Q_6 Laplacian
 matvec on 64^3 affine Cartesian cells of [0,1]^3 (385^3 = 57,066,625 nodes,
@@ -38,7 +38,8 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
-WORKLOADS = ("q6-fp64", "q6-fp32", "q4-128", "q4-16", "sipg-p5", "advect-p3", "varcoef-q8", "precomp-p3")
+WORKLOADS = ("q6-fp64", "q6-fp32", "q4-128", "q4-16", "sipg-p5", "advect-p3", "advect-p3-fp32", "varcoef-q8",
+             "precomp-p3")
 
 
 def log(*a):
@@ -238,11 +239,13 @@ def cont_entry(name, degree, n, prec, t, sched, profile=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=WORKLOADS, help="one workload's kernel, default schedule (profiling)")
+    ap.add_argument("--only", choices=WORKLOADS, help="one workload's kernel (profiling)")
+    ap.add_argument("--schedule", help="with --only: the schedule to run (the five integers of a bench line's "
+                                       "'schedule' entry), default: the operator's default schedule")
     ap.add_argument("--copy-probe", action="store_true",
                     help="plain copy kernel of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -492,47 +495,64 @@ def extras(args, configs):
 
 
 def only(args):
-    """one workload's kernel with its default schedule, for rocprofv3 passes
-    (every dispatch of the kernel is the measured one)"""
+    """one workload's kernel, for rocprofv3 passes (every dispatch of the
+    kernel is a measured one): the schedule given with --schedule (the
+    bench line's, so that profile and line describe the same launches) or
+    the operator's default"""
     import torch
     import sum_factorization_kernels_amd as sfk
     w = args.only
+
+    def schedule(op):
+        if args.schedule:
+            op.set_schedule(*[int(v) for v in args.schedule.split(",")])
+        return op.schedule
+
     if w in ("q6-fp64", "q6-fp32", "q4-128", "q4-16"):
         degree, n = (6, 64) if w.startswith("q6") else (4, 128 if w == "q4-128" else 16)
         prec = sfk.FP32 if w == "q6-fp32" else sfk.FP64
-        t, op, sched, _, _ = run_cont(degree, n, prec, args.steps, args.warmup, tune=False)
-        N = op.vector_size
-        log(f"{w}: schedule {sched}: {t * 1e6:.1f} us, {N / t / 1e9:.2f} GDoF/s")
+        op, _ = cont_op(degree, n, prec, tune=False)
+        sched = schedule(op)
+        dt = torch.float64 if prec == sfk.FP64 else torch.float32
+        x = torch.randn(op.vector_size, dtype=torch.float64, device="cuda").to(dt)
+        y = torch.empty_like(x)
+        t = timed(lambda: op.apply(x, y), args.steps, args.warmup)
+        log(f"{w}: schedule {sched}: {t * 1e6:.1f} us, {op.vector_size / t / 1e9:.2f} GDoF/s")
     elif w == "sipg-p5":
         op = sfk.Operator(sfk.DG_SIPG, 5, (32, 32, 32), geometry=sfk.GEOM_PER_Q)
         op.set_sine_deformation(0.1)
+        sched = schedule(op)
         u = torch.randn(op.vector_size, dtype=torch.float64, device="cuda")
         v = torch.empty_like(u)
         t = timed(lambda: op.apply(u, v), args.steps, args.warmup)
-        log(f"{w}: {t * 1e6:.1f} us, {op.vector_size / t / 1e9:.2f} GDoF/s")
-    elif w == "advect-p3":
-        o4 = sfk.Operator(sfk.DG_ADVECT, 3, (48, 48, 48), inv_jac=(48.0,) * 3, flags=sfk.FLAG_VARIABLE_VELOCITY)
+        log(f"{w}: schedule {sched}: {t * 1e6:.1f} us, {op.vector_size / t / 1e9:.2f} GDoF/s")
+    elif w in ("advect-p3", "advect-p3-fp32"):
+        prec, dt = (sfk.FP32, torch.float32) if w.endswith("fp32") else (sfk.FP64, torch.float64)
+        o4 = sfk.Operator(sfk.DG_ADVECT, 3, (48, 48, 48), inv_jac=(48.0,) * 3, precision=prec,
+                          flags=sfk.FLAG_VARIABLE_VELOCITY)
         o4.set_velocity(*advection_velocity(3, (48, 48, 48)))
-        u4 = torch.randn(o4.vector_size, dtype=torch.float64, device="cuda")
+        sched = schedule(o4)
+        u4 = torch.randn(o4.vector_size, dtype=torch.float64, device="cuda").to(dt)
         v4 = torch.empty_like(u4)
         t = timed(lambda: o4.apply(u4, v4), args.steps, args.warmup)
         ta = timed(lambda: o4.apply_transpose(u4, v4), args.steps, args.warmup)
-        log(f"{w}: forward {t * 1e6:.1f} us, adjoint {ta * 1e6:.1f} us")
+        log(f"{w}: schedule {sched}: forward {t * 1e6:.1f} us, adjoint {ta * 1e6:.1f} us")
     elif w == "varcoef-q8":
         o5 = sfk.Operator(sfk.CONT_LAPLACE, 8, (24, 24, 24), inv_jac=(24.0,) * 3, precision=sfk.FP32,
                           flags=sfk.FLAG_VARIABLE_COEF, n_rhs=8)
         o5.set_coefficient(np.exp(0.5 * np.random.default_rng(5).standard_normal(o5.n_cell_q)))
+        sched = schedule(o5)
         x5 = torch.randn(8 * o5.vector_size, dtype=torch.float32, device="cuda")
         y5 = torch.empty_like(x5)
         t = timed(lambda: o5.apply(x5, y5), args.steps, args.warmup)
-        log(f"{w}: {t * 1e3:.3f} ms")
+        log(f"{w}: schedule {sched}: {t * 1e3:.3f} ms")
     elif w == "precomp-p3":
         op = sfk.Operator(sfk.DG_PRECOMP, 3, (56, 50, 55))
-        op.set_schedule(4, 1, -1)
+        sched = schedule(op)
         x = torch.randn(op.vector_size, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
         t = timed(lambda: op.apply(x, y), args.steps, args.warmup)
-        log(f"{w}: {t * 1e6:.1f} us, {op.vector_size / t / 1e9:.2f} GDoF/s")
+        log(f"{w}: schedule {sched}: {t * 1e6:.1f} us, {op.vector_size / t / 1e9:.2f} GDoF/s")
 
 
 def advection_velocity(degree, nc):

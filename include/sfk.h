@@ -194,7 +194,12 @@ typedef struct sfk_schedule {
 
 /* Current schedule, with defaults resolved (cells_per_block > 0). */
 sfk_status sfk_op_get_schedule(const sfk_op *op, sfk_schedule *s);
-/* Sets the schedule; SFK_ERR_INVALID_ARG if cells_per_block exceeds
+/* Schedule of the tensor-product kernel's fused Chebyshev launches
+ * (sfk_cheby_step and the smoother of sfk_solve_cg_cheby on this operator):
+ * the one sfk_tune picked for them, else the matvec schedule. */
+sfk_status sfk_op_get_smoother_schedule(const sfk_op *op, sfk_schedule *s);
+/* Sets the schedule (of the matvec and of the Chebyshev launches);
+ * SFK_ERR_INVALID_ARG if cells_per_block exceeds
  * sfk_op_max_cells_per_block. */
 sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s);
 /* Largest cells_per_block of the operator (workgroup of at most 256 threads,
@@ -210,8 +215,10 @@ sfk_status sfk_op_tile_shape(const sfk_op *op, int shape, int *count, int *bx, i
  * cells_per_block in [1, max] x both block orders x, for the continuous
  * Laplacian, both scatters, and with a constant coefficient every tile shape
  * x z_cells (powers of two up to the cells in z) of the tensor-product
- * kernel.  Requires the operator's data (metric, coefficient, velocity) to
- * be set. */
+ * kernel; for that kernel the fused Chebyshev launch (three-term form, four
+ * vector streams) is timed over the same tile shapes, z_cells and orders and
+ * its fastest kept separately (sfk_op_get_smoother_schedule).  Requires the
+ * operator's data (metric, coefficient, velocity) to be set. */
 sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_seconds, void *stream);
 
 /* ---- solver (continuous Laplacian) ------------------------------------

@@ -103,6 +103,7 @@ def lib():
         L.sfk_vec_dot.argtypes = [ctypes.c_int, _vp, _vp, _ll, _dp, _vp]
         L.sfk_op_get_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
         L.sfk_op_set_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
+        L.sfk_op_get_smoother_schedule.argtypes = [_vp, ctypes.POINTER(Schedule)]
         L.sfk_op_max_cells_per_block.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
         L.sfk_op_tile_shape.argtypes = [_vp, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.sfk_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Schedule), _dp, _vp]
@@ -292,6 +293,15 @@ class Operator:
         """(cells_per_block, block_order, cont_scatter, tile_shape, z_cells) of the operator kernel"""
         sc = Schedule()
         check(lib().sfk_op_get_schedule(self.handle, ctypes.byref(sc)))
+        return sc.astuple()
+
+    @property
+    def smoother_schedule(self):
+        """schedule of the tensor-product kernel's fused Chebyshev launches
+        (cheby_step, the smoother of solve_cg_cheby): the one tune() picked
+        for them, else the matvec schedule"""
+        sc = Schedule()
+        check(lib().sfk_op_get_smoother_schedule(self.handle, ctypes.byref(sc)))
         return sc.astuple()
 
     def set_schedule(self, cells_per_block=0, block_order=1, cont_scatter=-1, tile_shape=0, z_cells=0):

@@ -147,7 +147,7 @@ int tp_default_zc(const sfk_op *op, int shape)
   return (int)zc;
 }
 
-void fill_tp_launch(const sfk_op *op, TpLaunch &L)
+void fill_tp_launch(const sfk_op *op, TpLaunch &L, bool cheby)
 {
   L.n = op->d.degree + 1;
   L.fp32 = op->d.precision == SFK_FP32;
@@ -155,9 +155,11 @@ void fill_tp_launch(const sfk_op *op, TpLaunch &L)
   for (int d = 0; d < 3; ++d) L.nc[d] = op->d.n_cells[d];
   L.nrhs = op->d.n_rhs;
   L.vstride = op->nvec;
-  L.shape = op->tp_shape;
-  L.zc = op->tp_zc > 0 ? op->tp_zc : tp_default_zc(op, op->tp_shape);
-  L.order = op->sched_order;
+  const bool own = cheby && op->cheby_shape >= 0;
+  L.shape = own ? op->cheby_shape : op->tp_shape;
+  const int zc = own ? op->cheby_zc : op->tp_zc;
+  L.zc = zc > 0 ? zc : tp_default_zc(op, L.shape);
+  L.order = own ? op->cheby_order : op->sched_order;
 }
 
 void fill_cell_launch(const sfk_op *op, CellLaunch &L)
@@ -542,7 +544,7 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
     if (uses_tp(op)) {
       // the update fused into the tensor-product kernel's stores
       TpLaunch T;
-      fill_tp_launch(op, T);
+      fill_tp_launch(op, T, true);
       T.cheby = 1;
       T.src = sol_old;
       T.dst = sol_new;

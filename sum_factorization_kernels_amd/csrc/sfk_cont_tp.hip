@@ -110,6 +110,22 @@ constexpr int tp_waves()
   return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
+// waves per SIMD the kernel is compiled for: the matvec (CHEBY 0) asks for
+// what the LDS allows; the Chebyshev forms hold more streams in registers
+// (prefetched rhs, diagonal, tm / previous iterate) and would spill at that
+// occupancy, so unless SFK_TP_CHEBY_WAVES is set they ask for one wave only
+// (the register count then sets the occupancy)
+#ifndef SFK_TP_CHEBY_WAVES
+#define SFK_TP_CHEBY_WAVES 0
+#endif
+template <int N, typename T, int BX, int BY, int CHEBY>
+constexpr int tp_waves_of()
+{
+  return CHEBY == 0 ? tp_waves<N, T, BX, BY>()
+                    : (SFK_TP_CHEBY_WAVES > 0 ? SFK_TP_CHEBY_WAVES
+                                              : (SFK_TP_CHEBY_WAVES < 0 ? tp_waves<N, T, BX, BY>() : 1));
+}
+
 // element at a 32-bit byte offset (per lane) from a uniform base pointer:
 // the load or store takes the scalar base and the vector offset directly
 // (no per-lane 64-bit address arithmetic)
@@ -179,7 +195,7 @@ __device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 
 template <int N, typename T, int BX, int BY, int CHEBY>
 __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS))
 #if SFK_TP_WAVES
-__attribute__((amdgpu_waves_per_eu(tp_waves<N, T, BX, BY>(), 8)))
+__attribute__((amdgpu_waves_per_eu(tp_waves_of<N, T, BX, BY, CHEBY>(), 8)))
 #endif
 cont_tp_kernel(const TpArgs<T> a)
 {

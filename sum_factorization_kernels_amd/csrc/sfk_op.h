@@ -25,6 +25,10 @@ struct sfk_op {
   // cells in z per workgroup (0 = automatic)
   int sched_cb = 0, sched_order = 1, cont_scatter = -1;
   int tp_shape = 0, tp_zc = 0;
+  // the tensor-product kernel's Chebyshev launches (sfk_cheby_step, the
+  // solver's smoother): tile shape, cells in z and block order chosen by
+  // sfk_tune for those launches (cheby_shape -1: the matvec's)
+  int cheby_shape = -1, cheby_zc = 0, cheby_order = 1;
   void *tptab = nullptr; // TpTab of the tensor-product kernel (constant coefficient)
   // precomputed DG Laplacian: neighbour coupling constants not both zero
   // (set with the device tables)
@@ -83,6 +87,7 @@ size_t elem_size(const sfk_op *op);
 sfk_status ensure_device_tables(sfk_op *op);
 // the continuous Laplacian's launches use the tensor-product kernel
 bool uses_tp(const sfk_op *op);
-void fill_tp_launch(const sfk_op *op, TpLaunch &L);
+// (cheby: the schedule of the fused Chebyshev launches)
+void fill_tp_launch(const sfk_op *op, TpLaunch &L, bool cheby = false);
 
 } // namespace sfk

@@ -286,7 +286,7 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         const double rho = 1. / (2. * sigma1 - rho_old);
         void *out = prev ? const_cast<void *>(prev) : other;
         TpLaunch T;
-        fill_tp_launch(S.sm, T);
+        fill_tp_launch(S.sm, T, true);
         T.cheby = 2;
         T.src = zc;
         T.prev = prev;

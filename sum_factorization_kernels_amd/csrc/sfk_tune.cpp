@@ -93,6 +93,19 @@ sfk_status sfk_op_get_schedule(const sfk_op *op, sfk_schedule *s)
   return SFK_OK;
 }
 
+sfk_status sfk_op_get_smoother_schedule(const sfk_op *op, sfk_schedule *s)
+{
+  if (!op || !s) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_op_get_smoother_schedule: null argument");
+  const sfk_status st = sfk_op_get_schedule(op, s);
+  if (st != SFK_OK) return st;
+  if (op->cheby_shape >= 0) {
+    s->block_order = op->cheby_order;
+    s->tile_shape = op->cheby_shape;
+    s->z_cells = op->cheby_zc;
+  }
+  return SFK_OK;
+}
+
 sfk_status sfk_op_tile_shape(const sfk_op *op, int shape, int *count, int *bx, int *by)
 {
   if (!op) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_op_tile_shape: null handle");
@@ -137,6 +150,7 @@ sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s)
   op->cont_scatter = s->cont_scatter;
   op->tp_shape = s->tile_shape;
   op->tp_zc = s->z_cells;
+  op->cheby_shape = -1; // the Chebyshev launches follow the new schedule
   return SFK_OK;
 }
 
@@ -151,6 +165,7 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
   sfk_status status = SFK_OK;
   const int saved_cb = op->sched_cb, saved_order = op->sched_order, saved_scatter = op->cont_scatter;
   const int saved_shape = op->tp_shape, saved_zc = op->tp_zc;
+  const int saved_cshape = op->cheby_shape, saved_czc = op->cheby_zc, saved_corder = op->cheby_order;
   sfk_schedule win{saved_cb > 0 ? saved_cb : default_cb(op), saved_order, saved_scatter, saved_shape, saved_zc};
   double win_t = -1.;
   if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
@@ -218,6 +233,61 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
       }
     }
   }
+  // the fused Chebyshev launch of the tensor-product kernel in the solver's
+  // three-term form (src x, rhs x, previous iterate pv, dst y: four
+  // streams, more registers than the matvec, so its best tile differs)
+  double cwin_t = -1.;
+  int cwin[3] = {-1, 0, 1};
+  if (status == SFK_OK && tp_ok(op)) {
+    void *pv = nullptr;
+    if (hipMalloc(&pv, bytes) != hipSuccess) {
+      set_error("sfk_tune: scratch allocation failed");
+      status = SFK_ERR_ALLOC;
+    }
+    if (status == SFK_OK) status = sfk_vec_fill(op->d.precision, pv, op->nvec * op->d.n_rhs, 0.0, stream);
+    if (status == SFK_OK) status = ensure_device_tables(op);
+    const int n = op->d.degree + 1, nz = op->d.n_cells[2];
+    std::vector<int> zcs;
+    for (int z = 1; z < nz; z *= 2) zcs.push_back(z);
+    zcs.push_back(nz);
+    for (int sh = 0; sh < tp_shape_count(n) && status == SFK_OK; ++sh) {
+      if (tp_lds_bytes(n, sh, op->d.precision == SFK_FP32) > (size_t)lds_limit()) continue;
+      for (int order = 0; order <= 1 && status == SFK_OK; ++order)
+        for (int zc : zcs) {
+          if (status != SFK_OK) break;
+          op->cheby_shape = sh;
+          op->cheby_order = order;
+          op->cheby_zc = zc;
+          TpLaunch T;
+          fill_tp_launch(op, T, true);
+          T.cheby = 2;
+          T.src = x;
+          T.rhs = x;
+          T.prev = pv;
+          T.dst = y;
+          T.c_np = 0.5;
+          T.c_tm = 0.25;
+          float ms = 0.f;
+          bool ok = launch_cont_tp(T, s) == hipSuccess && hipEventRecord(e0, s) == hipSuccess;
+          for (int r = 0; r < repeats && ok; ++r) ok = launch_cont_tp(T, s) == hipSuccess;
+          ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+               hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+          if (!ok) {
+            set_error("sfk_tune: Chebyshev launch or timing failed");
+            status = SFK_ERR_HIP;
+            break;
+          }
+          const double t = 1e-3 * ms / repeats;
+          if (cwin_t < 0. || t < cwin_t) {
+            cwin_t = t;
+            cwin[0] = sh;
+            cwin[1] = zc;
+            cwin[2] = order;
+          }
+        }
+    }
+    if (pv) (void)hipFree(pv);
+  }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (x) (void)hipFree(x);
@@ -228,8 +298,14 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
     op->cont_scatter = saved_scatter;
     op->tp_shape = saved_shape;
     op->tp_zc = saved_zc;
+    op->cheby_shape = saved_cshape;
+    op->cheby_zc = saved_czc;
+    op->cheby_order = saved_corder;
     return status != SFK_OK ? status : SFK_ERR_HIP;
   }
+  op->cheby_shape = cwin[0];
+  op->cheby_zc = cwin[1];
+  op->cheby_order = cwin[2];
   op->sched_cb = win.cells_per_block;
   op->sched_order = win.block_order;
   op->cont_scatter = win.cont_scatter;

@@ -203,6 +203,46 @@ def test_tune_picks_valid_schedule(cuda):
     assert rel_err(to_np(op.apply(to_dev(src))), ob.cont_laplacian(degree, nc, ij, src)) < TOL_FP64
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_tune_sets_smoother_schedule(cuda, prec):
+    """sfk_tune also picks the fused Chebyshev launches' tile, z_cells and
+    order; set_schedule makes them follow the matvec schedule again.  The
+    Chebyshev step with the tuned schedule matches the oracle, and the
+    solver (whose smoother uses that schedule) reaches 1e-10."""
+    import torch
+    degree, nc = 4, (6, 5, 8)
+    ij = (6.0, 5.0, 8.0)
+    p = sfk.FP32 if prec == "fp32" else sfk.FP64
+    tol = TOL_FP32 if prec == "fp32" else TOL_FP64
+    dt = torch.float32 if prec == "fp32" else torch.float64
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij, precision=p)
+    sched, _ = op.tune(repeats=2)
+    sm = op.smoother_schedule
+    count = op.tile_shape(0)[0]
+    assert sm[2] == sched[2] == 2
+    assert sm[1] in (0, 1) and 0 <= sm[3] < count and 1 <= sm[4] <= nc[2]
+    N = op.vector_size
+    rng = np.random.default_rng(4470)
+    u, rhs = rng.uniform(-1, 1, N), rng.standard_normal(N)
+    diag, tm = rng.uniform(0.5, 1.5, N), rng.standard_normal(N)
+    if prec == "fp32":
+        u, rhs, diag, tm = (v.astype(np.float32).astype(np.float64) for v in (u, rhs, diag, tm))
+    d_new, d_tm = _nan_vec(op), to_dev(tm, dt)
+    op.cheby_step(to_dev(rhs, dt), to_dev(u, dt), to_dev(diag, dt), d_new, d_tm, 0.8, 0.37)
+    r_tm = 0.37 * tm + 0.8 * diag * (ob.cont_laplacian(degree, nc, ij, u) - rhs)
+    assert rel_err(to_np(d_tm), r_tm) < tol
+    assert rel_err(to_np(d_new), u - r_tm) < tol
+    # the solver with this operator as the smoother
+    outer = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    b = outer.rhs_constant(1.0)
+    x = torch.zeros_like(b)
+    st = sfk.solve_cg_cheby(outer, op, b, x, rel_tol=1e-10, max_iter=500, cheby_degree=5, lanczos_steps=12)
+    assert st.final_residual <= 1e-10 * float(torch.linalg.norm(b))
+    # a schedule set by hand applies to the Chebyshev launches as well
+    op.set_schedule(0, 0, 2, 1, 2)
+    assert op.smoother_schedule == op.schedule
+
+
 @pytest.mark.parametrize("degree", [1, 3, 6])
 @pytest.mark.parametrize("scatter", [0, 1])
 @pytest.mark.parametrize("c_tm", [0.0, 0.37])
