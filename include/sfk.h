@@ -225,7 +225,7 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
  * Preconditioned CG in FP64 with a Chebyshev(degree) point-Jacobi
  * preconditioner whose bounds come from a Lanczos estimate of
  * lambda_max(D^{-1}A) (lanczos_steps FP64 steps, upper bound *= 1.1 (safety), lower =
- * upper/30).  The preconditioner runs on `smoother_op` (FP32 or FP64 operator
+ * upper/1000: a preconditioner acting on most of the spectrum).  The preconditioner runs on `smoother_op` (FP32 or FP64 operator
  * on the same mesh); the residual, inner products, Lanczos and the final
  * residual stay FP64 on `op` (which must be FP64).  x: initial guess in,
  * solution out; b: right-hand side (FP64, Dirichlet rows 0). */

@@ -344,59 +344,64 @@ cont_tp_kernel(const TpArgs<T> a)
       __builtin_nontemporal_store(v, &at(dst + zo, off));
     }
   };
-  // fused Chebyshev: rhs, diag, tmp and the input at this thread's node
-  // column for the p planes of a layer, loaded when the layer starts and in
-  // flight during phases Y and X (loaded when phase Z needs them, their
-  // latency was not hidden: a step took 4x a matvec)
+  // fused Chebyshev: rhs, diag, tmp (or the previous iterate) and the input
+  // at this thread's node column, plane k of a layer in q_*[k].  A rolling
+  // prefetch one layer ahead: the first layer's planes are loaded before the
+  // layer loop, and as soon as phase Z has written plane k of layer ez, the
+  // registers of plane k take the loads of layer ez + 1, in flight during the
+  // rest of phase Z and the next layer's phases Y and X (loaded at the start
+  // of their own layer, their latency was only partly hidden)
   constexpr int NCH = CHEBY ? p : 1;
   T q_r[NCH], q_d[CHEBY == 1 ? NCH : 1], q_t[NCH], q_s[NCH];
-  auto load_cheby = [&](int ez) {
-    if constexpr (CHEBY != 0) {
-      if (zone != 2) {
-        const long long c0 = coff / (unsigned)sizeof(T);
-#pragma unroll
-        for (int k = 0; k < p; ++k) {
-          const long long g = (long long)(ez * p + k) * plane + c0;
-          q_r[k] = a.rhs[g];
-          if constexpr (CHEBY == 1) {
-            q_d[k] = a.diag[g];
-            q_t[k] = a.c_tm != T(0) ? a.tm[g] : T(0);
-          } else {
-            q_t[k] = a.prev ? a.prev[g] : T(0);
-          }
-          q_s[k] = src[g];
-        }
-      }
+  auto load_plane = [&](int ez, int k) {
+    if constexpr (CHEBY == 1) {
+      const long long g = (long long)(ez * p + k) * plane + coff / (unsigned)sizeof(T);
+      q_r[k] = a.rhs[g];
+      q_d[k] = a.diag[g];
+      q_t[k] = a.c_tm != T(0) ? a.tm[g] : T(0);
+      q_s[k] = src[g];
+    } else if constexpr (CHEBY == 2) {
+      // uniform plane bases + this lane's 32-bit column offset
+      const long long zo = (long long)(ez * p + k) * plane;
+      q_r[k] = at(a.rhs + zo, coff);
+      q_t[k] = a.prev ? at(a.prev + zo, coff) : T(0);
+      q_s[k] = at(src + zo, coff);
     }
   };
-  // CHEBY == 2: the x/y part of this node column's diagonal, D = d1 dm[k] +
-  // d2 dk[2][k] in plane k of a cell (0 off the interior columns)
-  T d1 = T(0), d2 = T(0);
+  if constexpr (CHEBY != 0) {
+    if (zone != 2 && ez_begin < ez_end)
+#pragma unroll
+      for (int k = 0; k < p; ++k) load_plane(ez_begin, k);
+  }
+  // CHEBY == 2: the inverse diagonal of this node column in plane k of a
+  // cell, D = d1 dm[k] + d2 dk[2][k] with the x/y factors d1, d2 of the
+  // column (0 off the interior columns); the same in every layer, so formed
+  // once
+  T dinvk[CHEBY == 2 ? p : 1];
   if constexpr (CHEBY == 2) {
-    if (zone == 0 && X > 0 && Y > 0) {
-      const auto &tb = tptab<N, T>(a.tab);
-      const int ix = X % p, iy = Y % p;
-      d1 = tb.dk[0][ix] * tb.dm[iy] + tb.dm[ix] * tb.dk[1][iy];
-      d2 = tb.dm[ix] * tb.dm[iy];
-    }
+    const auto &tb = tptab<N, T>(a.tab);
+    const bool inner = zone == 0 && X > 0 && Y > 0;
+    const int ix = inner ? X % p : 0, iy = inner ? Y % p : 0;
+    const T d1 = tb.dk[0][ix] * tb.dm[iy] + tb.dm[ix] * tb.dk[1][iy];
+    const T d2 = tb.dm[ix] * tb.dm[iy];
+#pragma unroll
+    for (int k = 0; k < p; ++k) dinvk[k] = inner ? T(1) / (d1 * tb.dm[k] + d2 * tb.dk[2][k]) : T(0);
   }
   // output v of this thread's node in plane ez p + k (prefetched streams)
   auto put_own = [&](int ez, int k, T v) {
     if constexpr (CHEBY == 2) {
-      const long long g = (long long)(ez * p + k) * plane + coff / (unsigned)sizeof(T);
-      T dinv = T(0);
-      if (d2 != T(0) && ez * p + k > 0) {
-        const auto &tb = tptab<N, T>(a.tab);
-        dinv = T(1) / (d1 * tb.dm[k] + d2 * tb.dk[2][k]);
-      }
+      const long long zo = (long long)(ez * p + k) * plane; // uniform
+      const T dinv = ez * p + k > 0 ? dinvk[k] : T(0);
       const T t = a.c_tm * (q_t[k] - q_s[k]) + a.c_np * dinv * (v - q_r[k]);
-      dst[g] = q_s[k] - t;
+      at(dst + zo, coff) = q_s[k] - t;
+      if (ez + 1 < ez_end) load_plane(ez + 1, k);
     } else if constexpr (CHEBY == 1) {
       const long long g = (long long)(ez * p + k) * plane + coff / (unsigned)sizeof(T);
       const T res = v - q_r[k];
       const T t = (a.c_tm != T(0) ? a.c_tm * q_t[k] : T(0)) + a.c_np * q_d[k] * res;
       a.tm[g] = t;
       dst[g] = q_s[k] - t;
+      if (ez + 1 < ez_end) load_plane(ez + 1, k);
     } else {
       put(ez * p + k, coff, v);
     }
@@ -454,7 +459,6 @@ cont_tp_kernel(const TpArgs<T> a)
   // one layer: phase Y from the prefetched registers, the next layer's
   // loads, phases X and Z
   auto layer = [&](int ez, bool store, int ez_next) {
-    if (store) load_cheby(ez);
     store_y();
     __syncthreads();
     if (ez_next >= 0) load_y(ez_next, 1, all());

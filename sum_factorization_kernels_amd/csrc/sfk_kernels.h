@@ -182,13 +182,6 @@ hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, 
 hipError_t vec_mul_scaled(bool fp32, long long n, double c, const void *a, const void *b, void *z, hipStream_t s);
 // p = (double) zs + b p (zs FP32, p FP64)
 hipError_t vec_axpby_widen(long long n, const void *zs, double b, void *p, hipStream_t s);
-// result_dev[0..2] = z.r, z.q1, z.q2 (zs FP32, others FP64, q2 null: 0;
-// partials: 3*n_partials)
-hipError_t vec_widen_dot3(long long n, const void *zs, const void *r, const void *q1, const void *q2,
-                          double *result_dev, double *partials, int n_partials, hipStream_t s);
-// p2 = (double) zs + b1 p1 + b2 p2
-hipError_t vec_axpby2_widen(long long n, const void *zs, double b1, const void *p1, double b2, void *p2,
-                            hipStream_t s);
 hipError_t vec_cheby_update(bool fp32, long long n, const void *au, const void *rhs, const void *sol_old,
                             const void *diag, void *sol_new, void *tm, double c_np, double c_tm,
                             hipStream_t s);

@@ -8,7 +8,7 @@
 // k-1 of the k matvecs per iteration run in the smoother's precision.
 // Chebyshev bounds: lambda_max of D^{-1}A from `lanczos_steps` FP64 Lanczos
 // steps on D^{-1/2} A D^{-1/2} (largest Ritz value by bisection on the
-// tridiagonal matrix), times 1.1; lambda_min = lambda_max / 30.
+// tridiagonal matrix), times 1.1; lambda_min = lambda_max / 1000.
 // The Chebyshev recurrence (Saad, Iterative Methods for Sparse Linear
 // Systems, Alg. 12.1) is written in the fused update form of
 // EvaluationDGLaplacian::do_cheby_iter (legacy/include/evaluation_dg_laplacian.h
@@ -30,6 +30,7 @@ namespace sfk {
 namespace {
 
 constexpr int kPartials = 2048;
+constexpr double kChebyRange = 1000.; // lambda_max / lambda_min of the Chebyshev interval
 
 struct DevBuf {
   void *p = nullptr;
@@ -209,9 +210,9 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   // device work space: FP64 vectors, and smoother-precision vectors
   DevBuf red, d64, inv64, isq, r, w, lv0, lv1, lw, lt, e64;
   DevBuf rs, zs, qs, tms, inv_s;
-  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (3 * kPartials + 3)));
+  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (2 * kPartials + 2)));
   S.partials = (double *)red.p;
-  S.dres = S.partials + 3 * kPartials; // after the partials of three sums: three result slots
+  S.dres = S.partials + 2 * kPartials; // after the partials of two sums: two result slots
   for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
     SOLVER_HIP(hipMalloc(&bb->p, n8));
   for (DevBuf *bb : {&rs, &zs, &qs, &tms, &inv_s})
@@ -259,7 +260,11 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
     SOLVER_HIP(vec_axpby(false, n, 1. / beta, lw.p, 0., lv1.p, s));
   }
   const double lmax = 1.1 * tridiag_max_eig(al, be);
-  const double lmin = lmax / 30.;
+  // the polynomial is a preconditioner inside CG, not a multigrid smoother:
+  // it should act on most of the spectrum.  On the Q6 64^3 problem the FP64
+  // iteration count at degree 6 falls from 268 (lambda_max / 30) to 243
+  // (/100), 235 (/300) and 232 (/1000).
+  const double lmin = lmax / kChebyRange;
   S.st.lambda_max = lmax;
   S.st.lambda_min = lmin;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma1 = theta / delta;
@@ -369,25 +374,13 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   // PCG in FP64 (x, r, p, q, z and all inner products FP64).  The FP32
   // polynomial preconditioner is linear only up to FP32 rounding, so the
   // search direction uses the flexible (Polak-Ribiere) coefficient
-  // beta = z_new . (r_new - r_old) / (z_old . r_old) = -alpha (z_new . q) / rz,
-  // and with the FP32 smoother the new direction is A-orthogonalised against
-  // the last two directions instead of one (flexible CG with a truncation of
-  // two, Notay, SIAM J. Sci. Comput. 22 (2000) 1444-1460): the rounding of the
-  // FP32 smoother (1e-6 in the energy norm) otherwise costs the outer
-  // iteration 20-25 % more iterations than the FP64 smoother.
+  // beta = z_new . (r_new - r_old) / (z_old . r_old) = -alpha (z_new . q) / rz.
+  // (Orthogonalising against the last two directions as well, flexible CG
+  // with a truncation of two, did not reduce the iteration count on the
+  // Q6 64^3 problem and costs two more vector passes per iteration.)
   const double target = rel_tol * bnorm;
-#ifndef SFK_SOLVER_FCG2
-#define SFK_SOLVER_FCG2 1
-#endif
-  const bool fcg2 = SFK_SOLVER_FCG2 && f32;
   if (res > target) {
     void *z64 = nullptr, *pv = w.p, *qv = lt.p;
-    // FCG(2): the direction and its image from two iterations back (the
-    // Lanczos buffers are free by now)
-    void *p_old = lv0.p, *q_old = lv1.p;
-    bool have_prev = false;
-    double pq_prev = 0.;
-    if (fcg2) SOLVER_HIP(vec_fill(false, p_old, n, 0., s));
     void *rs32 = f32 ? rs.p : nullptr; // the CG update also writes (float) r
     if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rs.p, n, s));
     SOLVER_ST(precond(r.p, nullptr, &z64, nullptr));
@@ -422,26 +415,6 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         SOLVER_ST(precond(r.p, nullptr, &z64, nullptr));
         SOLVER_HIP(hipMemcpyAsync(pv, z64, n8, hipMemcpyDeviceToDevice, s));
         SOLVER_ST(S.dot(false, r.p, z64, rz));
-        // restart: no earlier direction to orthogonalise against
-        if (fcg2) SOLVER_HIP(vec_fill(false, p_old, n, 0., s));
-        have_prev = false;
-        continue;
-      }
-      if (fcg2) {
-        // z = P(r) in FP32; p_new = z - sum over the last two directions of
-        // (z.q_i / p_i.q_i) p_i, written over the older direction
-        void *z = zs.p, *spare = qs.p;
-        SOLVER_ST(cheby(rs.p, &z, &spare));
-        double d3[3];
-        SOLVER_HIP(vec_widen_dot3(n, z, r.p, qv, have_prev ? q_old : nullptr, S.dres, S.partials, kPartials, s));
-        SOLVER_ST(S.results(d3, 3));
-        const double b1 = -d3[1] / pq, b2 = have_prev ? -d3[2] / pq_prev : 0.;
-        SOLVER_HIP(vec_axpby2_widen(n, z, b1, pv, b2, p_old, s));
-        std::swap(pv, p_old);
-        std::swap(qv, q_old); // the next image goes over the older one
-        pq_prev = pq;
-        have_prev = true;
-        rz = d3[0];
         continue;
       }
       double zrq[2];

@@ -151,40 +151,6 @@ __global__ void axpby_kernel(long long n, T a, const T *x, T b, T *y)
 }
 
 // p = (double) zs + b p
-// z.r, z.q1, z.q2 (z FP32, the others FP64; q2 may be null: 0) with
-// per-block partial sums
-__global__ void __launch_bounds__(kThreads) widen_dot3_kernel(long long n, const float *zs, const double *r,
-                                                              const double *q1, const double *q2, double *partials)
-{
-  __shared__ double red[3][kThreads];
-  double a = 0., b = 0., c = 0.;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const double zi = double(zs[i]);
-    a += zi * r[i];
-    b += zi * q1[i];
-    if (q2) c += zi * q2[i];
-  }
-  red[0][threadIdx.x] = a;
-  red[1][threadIdx.x] = b;
-  red[2][threadIdx.x] = c;
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 3; ++k) partials[k * gridDim.x + blockIdx.x] = red[k][0];
-}
-
-// p2 = (double) zs + b1 p1 + b2 p2 (the new search direction over the one
-// before the last)
-__global__ void axpby2_widen_kernel(long long n, const float *zs, double b1, const double *p1, double b2, double *p2)
-{
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    p2[i] = double(zs[i]) + b1 * p1[i] + b2 * p2[i];
-}
-
 __global__ void axpby_widen_kernel(long long n, const float *zs, double b, double *p)
 {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -323,27 +289,6 @@ hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, c
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials + g, (int)g,
                      result_dev + 1);
-  return hipGetLastError();
-}
-
-hipError_t vec_widen_dot3(long long n, const void *zs, const void *r, const void *q1, const void *q2,
-                          double *result_dev, double *partials, int n_partials, hipStream_t s)
-{
-  unsigned g = grid_for(n, 8);
-  if ((int)g > n_partials) g = n_partials;
-  hipLaunchKernelGGL(widen_dot3_kernel, dim3(g), dim3(kThreads), 0, s, n, (const float *)zs, (const double *)r,
-                     (const double *)q1, (const double *)q2, partials);
-  for (int k = 0; k < 3; ++k)
-    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials + k * g, (int)g,
-                       result_dev + k);
-  return hipGetLastError();
-}
-
-hipError_t vec_axpby2_widen(long long n, const void *zs, double b1, const void *p1, double b2, void *p2,
-                            hipStream_t s)
-{
-  hipLaunchKernelGGL(axpby2_widen_kernel, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, (const float *)zs, b1,
-                     (const double *)p1, b2, (double *)p2);
   return hipGetLastError();
 }
 
