@@ -148,7 +148,10 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
  * it may alias src). */
 sfk_status sfk_rk_stage(sfk_op *op, const void *src, const void *upd, void *vec_np, void *vec_tm,
                         double c_np, double c_tm, void *stream);
-/* diag = diagonal of A (point-Jacobi); continuous Laplacian: 0 on Dirichlet rows */
+/* diag = diagonal of A (point-Jacobi); continuous Laplacian: 0 on Dirichlet
+ * rows.  SFK_ERR_UNSUPPORTED for advection and for a DG-SIPG or precomputed
+ * operator with one cell in some direction (the cell is then its own periodic
+ * neighbour). */
 sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream);
 
 /* Gradients (no reference counterpart):

@@ -394,9 +394,24 @@ class Graph:
             pass
 
 
+def _check_dev(t, dtypes, n=None, min_n=None, name="tensor"):
+    """a contiguous CUDA tensor of one of `dtypes` with n (or at least min_n)
+    elements: the C ABI takes raw device pointers and cannot check them"""
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous() or t.dtype not in dtypes:
+        raise ValueError(f"{name}: expected a contiguous CUDA tensor of dtype {' or '.join(map(str, dtypes))}")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"{name}: expected {n} elements, got {t.numel()}")
+    if min_n is not None and t.numel() < min_n:
+        raise ValueError(f"{name}: expected at least {min_n} elements, got {t.numel()}")
+
+
 def solve_cg_cheby(op, smoother, b, x, rel_tol=1e-10, max_iter=1000, cheby_degree=6, lanczos_steps=12,
                    stream=None):
     """FP64 PCG with a Chebyshev point-Jacobi preconditioner on `smoother`."""
+    import torch
+    _check_dev(b, (torch.float64,), op.vector_size, name="b")
+    _check_dev(x, (torch.float64,), op.vector_size, name="x")
     st = SolveStats()
     check(lib().sfk_solve_cg_cheby(op.handle, smoother.handle, ctypes.cast(b.data_ptr(), _dp),
                                    ctypes.cast(x.data_ptr(), _dp), rel_tol, max_iter, cheby_degree,
@@ -404,8 +419,13 @@ def solve_cg_cheby(op, smoother, b, x, rel_tol=1e-10, max_iter=1000, cheby_degre
     return st
 
 
+def _layout_size(degree, n_cells):
+    return int(n_cells[0]) * int(n_cells[1]) * int(n_cells[2]) * (int(degree) + 1) ** 3
+
+
 def layout_to_canonical(degree, n_cells, lanes, x, stream=None):
     import torch
+    _check_dev(x, (torch.float64,), _layout_size(degree, n_cells), name="x")
     out = torch.empty_like(x)
     nc = (ctypes.c_int * 3)(*n_cells)
     check(lib().sfk_layout_to_canonical(degree, nc, lanes, _dptr(x), _dptr(out), _stream(stream)))
@@ -414,6 +434,7 @@ def layout_to_canonical(degree, n_cells, lanes, x, stream=None):
 
 def layout_to_interleaved(degree, n_cells, lanes, x, stream=None):
     import torch
+    _check_dev(x, (torch.float64,), _layout_size(degree, n_cells), name="x")
     out = torch.empty_like(x)
     nc = (ctypes.c_int * 3)(*n_cells)
     check(lib().sfk_layout_to_interleaved(degree, nc, lanes, _dptr(x), _dptr(out), _stream(stream)))
@@ -424,6 +445,10 @@ def apply_1d(matrix, sym, forward, x, n_lines, line_stride, stride, stream=None)
     import numpy as np
     import torch
     nn = matrix.shape[0]
+    if n_lines < 0 or line_stride < 0 or stride < 1:
+        raise ValueError("apply_1d: n_lines, line_stride >= 0 and stride >= 1 expected")
+    extent = (n_lines - 1) * line_stride + (nn - 1) * stride + 1 if n_lines > 0 else 0
+    _check_dev(x, (torch.float64,), min_n=extent, name="x")
     out = torch.zeros_like(x)
     m = np.ascontiguousarray(matrix, dtype=np.float64).ravel()
     check(lib().sfk_apply_1d(nn, _np_ptr(m), sym, int(forward), n_lines, line_stride, stride, _dptr(x),
@@ -433,6 +458,8 @@ def apply_1d(matrix, sym, forward, x, n_lines, line_stride, stride, stream=None)
 
 def vec_dot(x, y, stream=None):
     import torch
+    _check_dev(x, (torch.float32, torch.float64), name="x")
+    _check_dev(y, (x.dtype,), x.numel(), name="y")
     r = ctypes.c_double()
     prec = FP32 if x.dtype == torch.float32 else FP64
     check(lib().sfk_vec_dot(prec, _dptr(x), _dptr(y), x.numel(), ctypes.byref(r), _stream(stream)))

@@ -608,6 +608,14 @@ sfk_status sfk_compute_diagonal(sfk_op *op, void *diag, void *stream)
     SFK_CHECK_HIP(launch_cell_diagonal(L, s));
     return SFK_OK;
   }
+  // periodic DG with one cell in a direction: the cell is its own neighbour
+  // there, and the diagonal kernels take the neighbour data as zero (the
+  // cell's face coupling to itself reaches the diagonal for the GLL basis and
+  // for low degrees of the Hermite-like basis)
+  const bool self_neighbour = d.n_cells[0] == 1 || d.n_cells[1] == 1 || d.n_cells[2] == 1;
+  SFK_REQUIRE(!((d.kind == SFK_DG_SIPG || d.kind == SFK_DG_PRECOMP) && self_neighbour), SFK_ERR_UNSUPPORTED,
+              "sfk_compute_diagonal: a DG mesh with one cell in a direction (the cell is its own periodic "
+              "neighbour) is not supported");
   if (d.kind == SFK_DG_SIPG) {
     SFK_REQUIRE(d.geometry == SFK_GEOM_PER_Q, SFK_ERR_UNSUPPORTED,
                 "DG-SIPG diagonal: per-q geometry (use the identity-mapped metric for Cartesian meshes)");

@@ -146,3 +146,32 @@ def test_no_oracle_in_product():
     """the shipped library never references the oracle or the reference"""
     data = open(sfk.LIB_PATH, "rb").read()
     assert b"sfko_" not in data and b"libsfkref" not in data and b"liboracle" not in data
+
+
+@pytest.mark.parametrize("kind,kw", [(sfk.DG_SIPG, dict(geometry=sfk.GEOM_PER_Q)), (sfk.DG_PRECOMP, {})])
+def test_dg_diagonal_rejects_self_neighbour(kind, kw):
+    """one cell in a direction: the cell is its own periodic neighbour, which
+    the diagonal kernels do not model; rejected before any device access (the
+    pointer below is host memory and is never read)"""
+    import ctypes
+    op = sfk.Operator(kind, 1, (3, 1, 2), **kw)
+    host = (ctypes.c_double * op.vector_size)()
+    st = sfk.lib().sfk_compute_diagonal(op.handle, ctypes.cast(host, ctypes.c_void_p), None)
+    assert st == 2  # SFK_ERR_UNSUPPORTED
+    assert "own periodic neighbour" in sfk.lib().sfk_last_error().decode()
+
+
+def test_raw_pointer_wrappers_validate_tensors():
+    """the wrappers over raw-pointer entry points reject tensors the C ABI
+    would read out of bounds (checked before any call into the library)"""
+    import torch
+    op = sfk.Operator(sfk.CONT_LAPLACE, 2, (2, 2, 2), inv_jac=(2.0, 2.0, 2.0))
+    cpu = torch.zeros(op.vector_size, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        sfk.solve_cg_cheby(op, op, cpu, cpu)
+    with pytest.raises(ValueError):
+        sfk.layout_to_canonical(2, (2, 2, 2), 2, torch.zeros(8 * 27, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        sfk.vec_dot(torch.zeros(4), torch.zeros(4))
+    with pytest.raises(ValueError):
+        sfk.apply_1d(np.eye(3), 1, True, torch.zeros(9, dtype=torch.float64), 3, 3, 1)

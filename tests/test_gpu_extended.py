@@ -141,3 +141,25 @@ def test_cg_chebyshev_solver(cuda, mixed):
     lam = np.linalg.eigvalsh(Ai / np.sqrt(np.outer(D, D)))
     assert st.lambda_max >= lam.max()
     assert st.lambda_max <= 1.25 * lam.max()
+
+
+def test_raw_pointer_wrappers_validate_device_tensors(cuda):
+    """wrong dtype or size of a CUDA tensor is rejected before the raw
+    pointer reaches the C ABI"""
+    import torch
+    op = sfk.Operator(sfk.CONT_LAPLACE, 2, (2, 2, 2), inv_jac=(2.0, 2.0, 2.0))
+    n = op.vector_size
+    b32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        sfk.solve_cg_cheby(op, op, b32, x)
+    with pytest.raises(ValueError):
+        sfk.solve_cg_cheby(op, op, x[:-1].contiguous(), x)
+    with pytest.raises(ValueError):
+        sfk.layout_to_canonical(2, (2, 2, 2), 2, b32)
+    with pytest.raises(ValueError):
+        sfk.vec_dot(x, x[:-1])
+    with pytest.raises(ValueError):
+        sfk.apply_1d(np.eye(3), 1, True, torch.zeros(8, dtype=torch.float64, device="cuda"), 3, 3, 1)
+    # valid calls still go through
+    assert sfk.vec_dot(x + 1.0, x + 2.0) == pytest.approx(2.0 * n)
