@@ -73,7 +73,7 @@ typedef enum { SFK_GEOM_CARTESIAN = 0, SFK_GEOM_PER_Q = 1 } sfk_geometry;
 
 typedef struct sfk_op_desc {
   int kind;          /* sfk_op_kind */
-  int degree;        /* polynomial degree p, 1..8 */
+  int degree;        /* polynomial degree p, 1..15 (precomputed DG: 1..8) */
   int precision;     /* sfk_precision: arithmetic and vector element type */
   int basis;         /* sfk_basis (SFK_CELL_LAPLACE); DG-SIPG is Hermite-like,
                         advection and continuous Q_p are GLL */

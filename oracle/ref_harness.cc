@@ -64,7 +64,7 @@ static int tables_dispatch(int kind, int p, double *values_eo, double *gradients
                            double *face_eo, double *inv_eo, double *hd)
 {
   if (p != degree) {
-    if constexpr (degree < 10)
+    if constexpr (degree < 15)
       return tables_dispatch<degree + 1>(kind, p, values_eo, gradients_eo, face_eo, inv_eo, hd);
     else
       return -1;
@@ -154,7 +154,7 @@ template <int degree>
 static int cell_dispatch(int p, int n_batches, const double *inv_jac, const double *src, double *dst)
 {
   if (p != degree) {
-    if constexpr (degree < 10)
+    if constexpr (degree < 15)
       return cell_dispatch<degree + 1>(p, n_batches, inv_jac, src, dst);
     else
       return -1;
@@ -184,7 +184,7 @@ static int dglap_dispatch(int p, const unsigned *nc, const double *inv_jac, int 
                           double *out_tm)
 {
   if (p != degree) {
-    if constexpr (degree < 10)
+    if constexpr (degree < 15)
       return dglap_dispatch<degree + 1>(p, nc, inv_jac, mode, sol_old, rhs, diag, tm_in, c_np, c_tm,
                                         out_new, out_tm);
     else
@@ -228,7 +228,7 @@ static int advect_dispatch(int p, const unsigned *nc, const double *inv_jac, int
                            double c_tm, double *out_np, double *out_tm)
 {
   if (p != degree) {
-    if constexpr (degree < 10)
+    if constexpr (degree < 15)
       return advect_dispatch<degree + 1>(p, nc, inv_jac, mode, src, upd, tm_in, c_np, c_tm, out_np,
                                          out_tm);
     else
@@ -272,7 +272,7 @@ static int advect_kat_dispatch(int p, const unsigned *nc, int n_steps, double *e
                                double *err_sol, double *t_end, double *inv_jac_out)
 {
   if (p != degree) {
-    if constexpr (degree < 10)
+    if constexpr (degree < 15)
       return advect_kat_dispatch<degree + 1>(p, nc, n_steps, err_deriv, err_sol, t_end, inv_jac_out);
     else
       return -1;
@@ -325,7 +325,7 @@ static int precomp_dispatch(int p, const unsigned *nc, const double *src, double
                             double *lap_eo, double *outer)
 {
   if (p != degree) {
-    if constexpr (degree < 10)
+    if constexpr (degree < 15)
       return precomp_dispatch<degree + 1>(p, nc, src, dst, mass_eo, lap_eo, outer);
     else
       return -1;

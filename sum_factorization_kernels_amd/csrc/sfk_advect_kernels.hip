@@ -471,7 +471,8 @@ static hipError_t dispatch_advect(const AdvectLaunch &L, hipStream_t s)
   switch (L.tables->n) {
 #define SFK_CASE(NN) \
   case NN: return launch_advect_n<NN, T>(L, s);
-    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9) \
+    SFK_CASE(10) SFK_CASE(11) SFK_CASE(12) SFK_CASE(13) SFK_CASE(14) SFK_CASE(15) SFK_CASE(16)
 #undef SFK_CASE
   default: return hipErrorInvalidValue;
   }
@@ -482,7 +483,8 @@ size_t advect_lds_per_cell(int n, bool fp32)
   switch (n) {
 #define SFK_CASE(NN) \
   case NN: return fp32 ? advect_lds_per_cell_n<NN, float>(false) : advect_lds_per_cell_n<NN, double>(false);
-    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9) \
+    SFK_CASE(10) SFK_CASE(11) SFK_CASE(12) SFK_CASE(13) SFK_CASE(14) SFK_CASE(15) SFK_CASE(16)
 #undef SFK_CASE
   default: return size_t(2 * n * n * n + 6 * n * n) * (fp32 ? 4 : 8);
   }

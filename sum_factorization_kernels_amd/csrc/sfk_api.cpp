@@ -91,7 +91,8 @@ static hipError_t upload_precomp(sfk_op *op, void **dev)
 // tensor-product kernel as well
 static bool has_tp(const sfk_op *op)
 {
-  return op->d.kind == SFK_CONT_LAPLACE && !(op->d.flags & SFK_FLAG_VARIABLE_COEF);
+  return op->d.kind == SFK_CONT_LAPLACE && !(op->d.flags & SFK_FLAG_VARIABLE_COEF) &&
+         op->d.degree <= kMaxDegreeTp;
 }
 
 sfk_status ensure_device_tables(sfk_op *op)
@@ -231,6 +232,8 @@ sfk_status sfk_op_create(const sfk_op_desc *desc, sfk_op **out)
     SFK_REQUIRE(d.basis == SFK_BASIS_HERMITE, SFK_ERR_UNSUPPORTED,
                 "precomputed DG Laplacian uses the Hermite-like basis");
     SFK_REQUIRE(d.geometry == SFK_GEOM_CARTESIAN, SFK_ERR_UNSUPPORTED, "precomputed DG Laplacian: Cartesian only");
+    SFK_REQUIRE(d.degree <= kMaxDegreeTp, SFK_ERR_UNSUPPORTED,
+                "precomputed DG Laplacian: degree 1.." + std::to_string(kMaxDegreeTp));
   }
   if (d.flags & SFK_FLAG_REF_PRECOMP_COUPLING)
     SFK_REQUIRE(d.kind == SFK_DG_PRECOMP, SFK_ERR_UNSUPPORTED,

@@ -820,7 +820,8 @@ static hipError_t dispatch_cell(const CellLaunch &L, hipStream_t s, bool diag)
   switch (L.tables->n) {
 #define SFK_CASE(NN) \
   case NN: return diag ? launch_diag_n<NN, T>(L, s) : launch_cell_n<NN, T>(L, s);
-    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9) \
+    SFK_CASE(10) SFK_CASE(11) SFK_CASE(12) SFK_CASE(13) SFK_CASE(14) SFK_CASE(15) SFK_CASE(16)
 #undef SFK_CASE
   default: return hipErrorInvalidValue;
   }
@@ -842,7 +843,8 @@ static hipError_t dispatch_coef_grad(const CoefGradLaunch &L, hipStream_t s)
   switch (L.tables->n) {
 #define SFK_CASE(NN) \
   case NN: return launch_coef_grad_n<NN, T>(L, s);
-    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9) \
+    SFK_CASE(10) SFK_CASE(11) SFK_CASE(12) SFK_CASE(13) SFK_CASE(14) SFK_CASE(15) SFK_CASE(16)
 #undef SFK_CASE
   default: return hipErrorInvalidValue;
   }

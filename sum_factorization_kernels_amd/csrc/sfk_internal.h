@@ -9,7 +9,10 @@
 
 namespace sfk {
 
-constexpr int kMaxDegree = 8;
+// degrees 1..15: (p+1)^2 <= 256 threads per cell.  The tensor-product
+// continuous kernel and the precomputed DG Laplacian stop at kMaxDegreeTp.
+constexpr int kMaxDegree = 15;
+constexpr int kMaxDegreeTp = 8;
 
 struct Tables1D {
   int degree = 0, n = 0, basis = 0;

@@ -474,7 +474,8 @@ static hipError_t dispatch_general(const SipgLaunch &L, hipStream_t s)
   switch (L.tables->n) {
 #define SFK_CASE(NN) \
   case NN: return launch_general_n<NN, T>(L, s);
-    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9) \
+    SFK_CASE(10) SFK_CASE(11) SFK_CASE(12) SFK_CASE(13) SFK_CASE(14) SFK_CASE(15) SFK_CASE(16)
 #undef SFK_CASE
   default: return hipErrorInvalidValue;
   }

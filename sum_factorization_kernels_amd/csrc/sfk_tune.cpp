@@ -31,7 +31,7 @@ int default_cb(const sfk_op *op)
   const int n = op->d.degree + 1;
   if (op->d.kind == SFK_DG_SIPG && op->d.geometry == SFK_GEOM_PER_Q) {
     static const int tab[10] = {0, 0, 64, 28, 12, 8, 3, 3, 2, 1};
-    return tab[n];
+    return n < 10 ? tab[n] : 1;
   }
   return kMaxThreads / (n * n) > 0 ? kMaxThreads / (n * n) : 1;
 }
@@ -65,7 +65,7 @@ int lds_limit()
 
 bool tp_ok(const sfk_op *op)
 {
-  return op->d.kind == SFK_CONT_LAPLACE && !(op->d.flags & SFK_FLAG_VARIABLE_COEF);
+  return op->d.kind == SFK_CONT_LAPLACE && !(op->d.flags & SFK_FLAG_VARIABLE_COEF) && op->d.degree <= kMaxDegreeTp;
 }
 
 int max_cb(const sfk_op *op)

@@ -367,7 +367,8 @@ hipError_t upload_device_tables(const Tables1D &t, bool fp32, void **dev)
   switch (t.n) {
 #define SFK_CASE(NN) \
   case NN: return fp32 ? upload_devtab<NN, float>(t, dev) : upload_devtab<NN, double>(t, dev);
-    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9) \
+    SFK_CASE(10) SFK_CASE(11) SFK_CASE(12) SFK_CASE(13) SFK_CASE(14) SFK_CASE(15) SFK_CASE(16)
 #undef SFK_CASE
   default: return hipErrorInvalidValue;
   }
@@ -383,7 +384,8 @@ hipError_t apply_1d(int nn, const std::vector<double> &M, int sym, bool forward,
     hipLaunchKernelGGL(apply1d_kernel<NN>, dim3(g), dim3(kThreads), 0, s, make_eo<NN, double>(M, forward), sym, \
                        n_lines, line_stride, stride, in, out);                                                \
     break;
-    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9)
+    SFK_CASE(2) SFK_CASE(3) SFK_CASE(4) SFK_CASE(5) SFK_CASE(6) SFK_CASE(7) SFK_CASE(8) SFK_CASE(9) \
+    SFK_CASE(10) SFK_CASE(11) SFK_CASE(12) SFK_CASE(13) SFK_CASE(14) SFK_CASE(15) SFK_CASE(16)
 #undef SFK_CASE
   default: return hipErrorInvalidValue;
   }

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <iomanip>
 #include <iostream>
+#include <type_traits>
 #include <limits>
 
 #include "sfk_evaluators.h"
@@ -70,6 +71,20 @@ static void run_program(unsigned long vector_size_guess, unsigned n_tests, bool 
             << " GFLOP/s: " << 1. * evaluator.n_elements() * ops_cell * n_tests / best * 1e-9 << std::endl;
 }
 
+// runs f(std::integral_constant<int, degree>) for a run-time degree in 1..MAX
+template <int MAX, int D = 1, typename F>
+static bool for_degree(int degree, F &&f)
+{
+  if (degree == D) {
+    f(std::integral_constant<int, D>());
+    return true;
+  }
+  if constexpr (D < MAX)
+    return for_degree<MAX, D + 1>(degree, f);
+  else
+    return false;
+}
+
 int main(int argc, char **argv)
 {
   const unsigned long vector_size = argc > 1 ? std::atol(argv[1]) : 300000;
@@ -78,14 +93,10 @@ int main(int argc, char **argv)
   const bool graph = argc > 4 && std::atoi(argv[4]) != 0;
   std::cout << sfk_version() << std::endl;
   try {
-    if (only == -1 || only == 1) run_program<1>(vector_size, n_tests, graph);
-    if (only == -1 || only == 2) run_program<2>(vector_size, n_tests, graph);
-    if (only == -1 || only == 3) run_program<3>(vector_size, n_tests, graph);
-    if (only == -1 || only == 4) run_program<4>(vector_size, n_tests, graph);
-    if (only == -1 || only == 5) run_program<5>(vector_size, n_tests, graph);
-    if (only == -1 || only == 6) run_program<6>(vector_size, n_tests, graph);
-    if (only == -1 || only == 7) run_program<7>(vector_size, n_tests, graph);
-    if (only == -1 || only == 8) run_program<8>(vector_size, n_tests, graph);
+    // degrees 1..14, as legacy/src/test_dg_advect.cc:18-19
+    for (int degree = 1; degree <= 14; ++degree)
+      if (only == -1 || only == degree)
+        for_degree<14>(degree, [&](auto d) { run_program<decltype(d)::value>(vector_size, n_tests, graph); });
   } catch (const sfk::Error &e) {
     std::cerr << "error: " << e.what() << std::endl;
     return 2;

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <iomanip>
 #include <iostream>
+#include <type_traits>
 #include <limits>
 
 #include "sfk_evaluators.h"
@@ -80,6 +81,22 @@ static void run_program(unsigned long vector_size_guess, unsigned n_tests, unsig
     report("VU", time_best([&] { evaluator.emulate_cheby_vector_updates(); }), 7, 6 * evaluator.dofs_per_cell);
 }
 
+// runs f(std::integral_constant<int, degree>) for a run-time degree in 1..MAX
+// (the reference's drivers recurse over their degree range the same way,
+// RunTime<dim, degree, max_degree>)
+template <int MAX, int D = 1, typename F>
+static bool for_degree(int degree, F &&f)
+{
+  if (degree == D) {
+    f(std::integral_constant<int, D>());
+    return true;
+  }
+  if constexpr (D < MAX)
+    return for_degree<MAX, D + 1>(degree, f);
+  else
+    return false;
+}
+
 int main(int argc, char **argv)
 {
   int degree = argc > 1 ? std::atoi(argv[1]) : 4;
@@ -88,16 +105,10 @@ int main(int argc, char **argv)
   const unsigned variants = argc > 4 ? std::atoi(argv[4]) : 2;
   std::cout << sfk_version() << std::endl;
   try {
-    switch (degree) {
-    case 1: run_program<1>(vector_size, n_tests, variants); break;
-    case 2: run_program<2>(vector_size, n_tests, variants); break;
-    case 3: run_program<3>(vector_size, n_tests, variants); break;
-    case 4: run_program<4>(vector_size, n_tests, variants); break;
-    case 5: run_program<5>(vector_size, n_tests, variants); break;
-    case 6: run_program<6>(vector_size, n_tests, variants); break;
-    case 7: run_program<7>(vector_size, n_tests, variants); break;
-    case 8: run_program<8>(vector_size, n_tests, variants); break;
-    default: std::cerr << "degree must be in 1..8\n"; return 1;
+    const bool known = for_degree<12>(degree, [&](auto d) { run_program<decltype(d)::value>(vector_size, n_tests, variants); });
+    if (!known) {
+      std::cerr << "degree must be in 1..12\n";
+      return 1;
     }
   } catch (const sfk::Error &e) {
     std::cerr << "error: " << e.what() << std::endl;

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <iomanip>
 #include <iostream>
+#include <type_traits>
 #include <limits>
 
 #include "sfk_evaluators.h"
@@ -40,6 +41,22 @@ static void run_program(unsigned n_tests, unsigned long vector_size, int basis)
             << "  ops/dof  " << ops / n_dofs << std::endl;
 }
 
+// runs f(std::integral_constant<int, degree>) for a run-time degree in 1..MAX
+// (the reference's drivers recurse over their degree range the same way,
+// RunTime<dim, degree, max_degree>)
+template <int MAX, int D = 1, typename F>
+static bool for_degree(int degree, F &&f)
+{
+  if (degree == D) {
+    f(std::integral_constant<int, D>());
+    return true;
+  }
+  if constexpr (D < MAX)
+    return for_degree<MAX, D + 1>(degree, f);
+  else
+    return false;
+}
+
 int main(int argc, char **argv)
 {
   const int degree = argc > 1 ? std::atoi(argv[1]) : 4;
@@ -48,16 +65,10 @@ int main(int argc, char **argv)
   const int basis = argc > 4 ? std::atoi(argv[4]) : 0;
   std::cout << sfk_version() << std::endl;
   try {
-    switch (degree) {
-    case 1: run_program<1>(n_tests, vector_size, basis); break;
-    case 2: run_program<2>(n_tests, vector_size, basis); break;
-    case 3: run_program<3>(n_tests, vector_size, basis); break;
-    case 4: run_program<4>(n_tests, vector_size, basis); break;
-    case 5: run_program<5>(n_tests, vector_size, basis); break;
-    case 6: run_program<6>(n_tests, vector_size, basis); break;
-    case 7: run_program<7>(n_tests, vector_size, basis); break;
-    case 8: run_program<8>(n_tests, vector_size, basis); break;
-    default: std::cerr << "degree must be in 1..8\n"; return 1;
+    const bool known = for_degree<15>(degree, [&](auto d) { run_program<decltype(d)::value>(n_tests, vector_size, basis); });
+    if (!known) {
+      std::cerr << "degree must be in 1..15\n";
+      return 1;
     }
   } catch (const sfk::Error &e) {
     std::cerr << "error: " << e.what() << std::endl;

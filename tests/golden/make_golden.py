@@ -2,7 +2,9 @@
 implementation, compiled from the reference headers by oracle/Makefile into
 oracle/_ref/libsfkref.so (this container only; the reference does not travel).
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [--high]
+
+(--high: only the fixtures of the degrees above 8)
 
 Inputs are regenerated from the stored seeds (numpy default_rng); only the
 reference outputs are stored, in the canonical layout (cell-major, x
@@ -162,5 +164,65 @@ def main():
     save("advect_kat.npz", **kat)
 
 
+def high_degrees():
+    """fixtures of the degrees above 8 (the reference drivers run the DG
+    Laplacian to degree 12, advection to 14 and the cell Laplacian beyond):
+    one x row of L cells each, to keep the files small"""
+    if not ob.ref_available():
+        raise SystemExit("oracle/_ref/libsfkref.so missing: run `make -C oracle ref`")
+    R = ob.ref()
+    L = ob.ref_lanes()
+    nc = (L, 1, 1)
+    for p in (10, 12, 15):
+        n3 = (p + 1) ** 3
+        seed = 10 + p
+        src = np.random.default_rng(seed).standard_normal(L * n3)
+        out_i = ob.ref_cell_laplacian(p, ob.to_interleaved(p, nc, L, src), CELL_IJ)
+        save(f"cell_laplacian_p{p}.npz", seed=seed, n_cells=np.array(nc), inv_jac=np.array(CELL_IJ),
+             out=ob.to_canonical(p, nc, L, out_i))
+    for p in (10, 12):
+        N = (p + 1) ** 3 * L
+        seed = 20 + p
+        rng = np.random.default_rng(seed)
+        src, rhs, diag, tm = rng.standard_normal(N), rng.standard_normal(N), rng.uniform(0.5, 1.5, N), \
+            rng.standard_normal(N)
+        I = lambda x: ob.to_interleaved(p, nc, L, x)  # noqa: E731
+        C = lambda x: ob.to_canonical(p, nc, L, x)  # noqa: E731
+        mv, _ = ob.ref_dg_laplacian(p, nc, CELL_IJ, I(src))
+        new, tmo = ob.ref_dg_laplacian(p, nc, CELL_IJ, I(src), I(rhs), I(diag), I(tm), 0.7, 0.3, mode=1)
+        mask = np.ones((nc[2], nc[1], nc[0]), dtype=np.int8)
+        mask[:, :, nc[0] - 1] = 0
+        save(f"dg_laplacian_p{p}.npz", seed=seed, n_cells=np.array(nc), inv_jac=np.array(CELL_IJ), c_np=0.7,
+             c_tm=0.3, matvec=C(mv), cheby_new=C(new), cheby_tm=C(tmo), mask=mask.ravel())
+    for p in (11, 14):
+        N = (p + 1) ** 3 * L
+        seed = 30 + p
+        rng = np.random.default_rng(seed)
+        src, upd, tm = rng.standard_normal(N), rng.standard_normal(N), rng.standard_normal(N)
+        I = lambda x: ob.to_interleaved(p, nc, L, x)  # noqa: E731
+        C = lambda x: ob.to_canonical(p, nc, L, x)  # noqa: E731
+        mv, _ = ob.ref_dg_advect(p, nc, None, I(src))
+        vnp, vtm = ob.ref_dg_advect(p, nc, None, I(src), I(upd), I(tm), 0.3, 0.2, mode=1)
+        save(f"dg_advect_p{p}.npz", seed=seed, n_cells=np.array(nc),
+             inv_jac=np.array([nc[0] / 2.0, nc[1] / 2.0, nc[2] / 2.0]), velocity=np.array([1.0, 0.15, -0.05]),
+             c_np=0.3, c_tm=0.2, matvec=C(mv), stage_np=C(vnp), stage_tm=C(vtm))
+    kat = {}
+    nck = (L, 4, 4)
+    for p in (10, 14):
+        ncu = (ob.ctypes.c_uint * 3)(*nck)
+        ed, es, te, ij = np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(3)
+        n_steps = 10
+        rc = R.ref_advect_kat(p, ncu, n_steps, ob._ptr(ed), ob._ptr(es), ob._ptr(te), ob._ptr(ij))
+        assert rc == 0
+        kat[f"p{p}"] = np.array([ed[0], es[0], te[0], n_steps])
+        kat[f"inv_jac_p{p}"] = ij
+    kat["n_cells"] = np.array(nck)
+    save("advect_kat_high.npz", **kat)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--high"]:
+        high_degrees()
+    else:
+        main()
+        high_degrees()

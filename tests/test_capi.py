@@ -44,7 +44,7 @@ def test_version_and_error_string():
 
 
 @pytest.mark.parametrize("bad", [
-    dict(degree=0), dict(degree=9), dict(n_cells=(0, 1, 1)), dict(inv_jac=(1.0, -1.0, 1.0)),
+    dict(degree=0), dict(degree=16), dict(n_cells=(0, 1, 1)), dict(inv_jac=(1.0, -1.0, 1.0)),
     dict(inv_jac=(1.0, float("nan"), 1.0)), dict(n_rhs=0), dict(precision=7), dict(kind=9),
 ])
 def test_descriptor_validation(bad):
@@ -175,3 +175,16 @@ def test_raw_pointer_wrappers_validate_tensors():
         sfk.vec_dot(torch.zeros(4), torch.zeros(4))
     with pytest.raises(ValueError):
         sfk.apply_1d(np.eye(3), 1, True, torch.zeros(9, dtype=torch.float64), 3, 3, 1)
+
+
+def test_degree_limits():
+    """degrees 1..15 ((p+1)^2 <= 256 threads per cell); the precomputed DG
+    Laplacian and the tensor-product continuous kernel stop at 8"""
+    sfk.Operator(sfk.CELL_LAPLACE, 15, (1, 1, 1))
+    with pytest.raises(sfk.SfkError):
+        sfk.Operator(sfk.CELL_LAPLACE, 16, (1, 1, 1))
+    with pytest.raises(sfk.SfkError):
+        sfk.Operator(sfk.DG_PRECOMP, 9, (2, 2, 2))
+    op = sfk.Operator(sfk.CONT_LAPLACE, 9, (2, 2, 2), inv_jac=(2.0, 2.0, 2.0))
+    with pytest.raises(sfk.SfkError):
+        op.set_schedule(0, 1, 2)

@@ -155,6 +155,10 @@ DRIVER_RUNS = [
     ("test_program", ["4", "3", "100000"], "DoFs/s"),
     ("test_dg_advect", ["100000", "2", "3", "0"], "Error against analytic solution"),
     ("test_dg_advect", ["100000", "2", "3", "1"], "(graph)"),
+    # the reference drivers' top degrees (test_dg_integrate.cc:23, test_dg_advect.cc:19)
+    ("test_dg_integrate", ["12", "300000", "2"], "MV degree 12"),
+    ("test_dg_advect", ["200000", "2", "14", "0"], "Error against analytic solution"),
+    ("test_program", ["15", "2", "200000"], "DoFs/s"),
 ]
 
 

@@ -19,7 +19,7 @@ def load(name):
     return np.load(os.path.join(GOLD, name))
 
 
-@pytest.mark.parametrize("degree", [1, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("degree", [1, 3, 4, 5, 6, 7, 8, 10, 12, 15])
 def test_cell_laplacian_golden(cuda, degree):
     g = load(f"cell_laplacian_p{degree}.npz")
     n3 = (degree + 1) ** 3
@@ -29,7 +29,7 @@ def test_cell_laplacian_golden(cuda, degree):
     assert rel_err(to_np(op.apply(to_dev(src))), g["out"]) < TOL
 
 
-@pytest.mark.parametrize("degree", [1, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("degree", [1, 3, 4, 5, 6, 7, 8, 10, 12])
 def test_dg_sipg_reference_mode_golden(cuda, degree):
     g = load(f"dg_laplacian_p{degree}.npz")
     nc = [int(x) for x in g["n_cells"]]
@@ -47,7 +47,7 @@ def test_dg_sipg_reference_mode_golden(cuda, degree):
     assert rel_err(to_np(d_tm)[mask], g["cheby_tm"][mask]) < TOL
 
 
-@pytest.mark.parametrize("degree", range(1, 9))
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 5, 6, 7, 8, 11, 14])
 def test_dg_advect_golden(cuda, degree):
     g = load(f"dg_advect_p{degree}.npz")
     nc = [int(x) for x in g["n_cells"]]
@@ -62,11 +62,11 @@ def test_dg_advect_golden(cuda, degree):
     assert rel_err(to_np(d_tm), g["stage_tm"]) < TOL
 
 
-@pytest.mark.parametrize("degree", range(1, 8))
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 5, 6, 7, 10, 14])
 def test_advect_derivative_kat_gpu(cuda, degree):
     """the reference's verify_derivative value (evaluation_dg_advect.h:221-251)"""
-    from test_oracle_golden import exact_solution, gll_nodal
-    k = load("advect_kat.npz")
+    from test_oracle_golden import _kat, exact_solution, gll_nodal
+    k = _kat(degree)
     nc = [int(x) for x in k["n_cells"]]
     ij = k[f"inv_jac_p{degree}"]
     u = gll_nodal(degree, nc, ij, lambda x, y, z: exact_solution(x, y, z, 0.0))
@@ -97,12 +97,12 @@ def lsrk_time_loop(op, u0, n_steps, dt):
     return new, t
 
 
-@pytest.mark.parametrize("degree", range(1, 8))
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 5, 6, 7, 10, 14])
 def test_advect_time_loop_kat_gpu(cuda, degree):
-    """error against the travelling wave after 20 LSRK steps equals the
-    reference's compute_max_error value (evaluation_dg_advect.h:253-283)"""
-    from test_oracle_golden import exact_solution, gll_nodal
-    k = load("advect_kat.npz")
+    """error against the travelling wave after the reference's LSRK steps
+    equals its compute_max_error value (evaluation_dg_advect.h:253-283)"""
+    from test_oracle_golden import _kat, exact_solution, gll_nodal
+    k = _kat(degree)
     nc = [int(x) for x in k["n_cells"]]
     ij = k[f"inv_jac_p{degree}"]
     _, err_sol, t_end, n_steps = k[f"p{degree}"]
