@@ -39,7 +39,7 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
 WORKLOADS = ("q6-fp64", "q6-fp32", "q4-128", "q4-16", "sipg-p5", "advect-p3", "advect-p3-fp32", "varcoef-q8",
-             "precomp-p3")
+             "precomp-p3", "cg-q6")
 
 
 def log(*a):
@@ -472,7 +472,8 @@ def extras(args, configs):
     nq5 = o5.n_cell_q
     configs["c5_varcoef_q8_24cube_8rhs_fp32"] = {
         "gdofs": round(8 * N5 / t5 / 1e9, 3), "schedule": list(sched5), "apply_ms": round(t5 * 1e3, 4),
-        "roofline": roofline(8 * 2 * 4 * N5 + 4 * nq5, t5, flops=8 * 13824 * 99873, fp32=True),
+        "roofline": roofline(8 * 2 * 4 * N5 + 4 * nq5, t5, read_traffic("cont_varcoef_q8_24cube_8rhs_fp32_pmc.txt"),
+                             flops=8 * 13824 * 99873, fp32=True),
         "coef_grad_ms": round(t5g * 1e3, 4),
         "coef_grad_roofline": roofline(8 * 2 * 4 * N5 + 4 * nq5, t5g)}
     log(f"var-coef Q8 24^3 x8 FP32: {t5 * 1e3:.3f} ms, {8 * N5 / t5 / 1e9:.2f} GDoF/s")
@@ -546,6 +547,18 @@ def only(args):
         y5 = torch.empty_like(x5)
         t = timed(lambda: o5.apply(x5, y5), args.steps, args.warmup)
         log(f"{w}: schedule {sched}: {t * 1e3:.3f} ms")
+    elif w == "cg-q6":
+        # one FP64 and one mixed CG-Chebyshev(6) solve of the Q6 64^3 problem
+        # to 1e-10, operators tuned first (as in the full run)
+        o3, _ = cont_op(6, 64, sfk.FP64)
+        s32, _ = cont_op(6, 64, sfk.FP32)
+        b = o3.rhs_constant(1.0)
+        x = torch.zeros_like(b)
+        for name, sm in (("fp64", o3), ("mixed", s32)):
+            x.zero_()
+            st = sfk.solve_cg_cheby(o3, sm, b, x, 1e-10, 3000, 6, 12)
+            log(f"{w} {name}: {st.seconds:.3f} s, {st.iterations} iterations, smoother schedule "
+                f"{sm.smoother_schedule}")
     elif w == "precomp-p3":
         op = sfk.Operator(sfk.DG_PRECOMP, 3, (56, 50, 55))
         sched = schedule(op)
