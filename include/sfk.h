@@ -221,7 +221,13 @@ sfk_status sfk_op_tile_shape(const sfk_op *op, int shape, int *count, int *bx, i
  * kernel; for that kernel the fused Chebyshev launch (three-term form, four
  * vector streams) is timed over the same tile shapes, z_cells and orders and
  * its fastest kept separately (sfk_op_get_smoother_schedule).  Requires the
- * operator's data (metric, coefficient, velocity) to be set. */
+ * operator's data (metric, coefficient, velocity) to be set.
+ * Tuning cache: when the environment variable SFK_TUNE_CACHE names a file,
+ * an operator whose key (device architecture and CU count, kind, degree,
+ * precision, basis, geometry, flags, cells, right-hand sides) has a line
+ * there gets that line's schedules and time without any timing run (a line
+ * that no longer validates is ignored); otherwise the result of the timing
+ * runs is appended to the file. */
 sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_seconds, void *stream);
 
 /* ---- solver (continuous Laplacian) ------------------------------------

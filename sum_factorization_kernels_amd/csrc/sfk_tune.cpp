@@ -5,6 +5,10 @@
 // continuous scatter) and sfk_tune measures every candidate.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -76,6 +80,70 @@ int max_cb(const sfk_op *op)
   const int by_lds = (int)(size_t(lds_max) / lds_per_cell(op));
   if (by_lds < m) m = by_lds;
   return m > 0 ? m : 1;
+}
+
+// ---- tuning cache: the file named by SFK_TUNE_CACHE holds one line per
+// tuned operator, "<key> <cb> <order> <scatter> <shape> <zc> <cheby shape>
+// <cheby zc> <cheby order> <seconds>", the key naming the device and every
+// descriptor field the kernels and their timing depend on.  Lines are
+// appended; the last line of a key wins.
+std::mutex g_cache_mutex;
+
+std::string cache_key(const sfk_op *op)
+{
+  std::string dev = "nodevice";
+  int id = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&id) == hipSuccess && hipGetDeviceProperties(&prop, id) == hipSuccess) {
+    dev = prop.gcnArchName;
+    dev += ":" + std::to_string(prop.multiProcessorCount);
+  }
+  for (char &c : dev)
+    if (c == ' ' || c == '\t') c = '_';
+  const sfk_op_desc &d = op->d;
+  std::ostringstream k;
+  k << dev << "/k" << d.kind << "/p" << d.degree << "/f" << d.precision << "/b" << d.basis << "/g" << d.geometry
+    << "/fl" << d.flags << "/n" << d.n_cells[0] << "x" << d.n_cells[1] << "x" << d.n_cells[2] << "/r" << d.n_rhs;
+  return k.str();
+}
+
+struct CacheEntry {
+  sfk_schedule s;
+  int cheby[3];
+  double seconds;
+};
+
+bool cache_lookup(const char *path, const std::string &key, CacheEntry &out)
+{
+  std::lock_guard<std::mutex> lock(g_cache_mutex);
+  FILE *f = std::fopen(path, "r");
+  if (!f) return false;
+  bool found = false;
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    std::istringstream in(line);
+    std::string k;
+    CacheEntry e{};
+    if (!(in >> k >> e.s.cells_per_block >> e.s.block_order >> e.s.cont_scatter >> e.s.tile_shape >>
+          e.s.z_cells >> e.cheby[0] >> e.cheby[1] >> e.cheby[2] >> e.seconds))
+      continue; // not a cache line
+    if (k == key) {
+      out = e;
+      found = true;
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+
+void cache_store(const char *path, const std::string &key, const CacheEntry &e)
+{
+  std::lock_guard<std::mutex> lock(g_cache_mutex);
+  FILE *f = std::fopen(path, "a");
+  if (!f) return; // an unwritable cache only costs the next run its tuning
+  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %.9g\n", key.c_str(), e.s.cells_per_block, e.s.block_order,
+               e.s.cont_scatter, e.s.tile_shape, e.s.z_cells, e.cheby[0], e.cheby[1], e.cheby[2], e.seconds);
+  std::fclose(f);
 }
 
 } // namespace
@@ -158,6 +226,26 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
 {
   if (!op) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_tune: null handle");
   if (repeats < 1 || repeats > 10000) TUNE_FAIL(SFK_ERR_INVALID_ARG, "sfk_tune: repeats must be in 1..10000");
+  const char *cache = std::getenv("SFK_TUNE_CACHE");
+  const std::string key = cache && *cache ? cache_key(op) : std::string();
+  if (!key.empty()) {
+    CacheEntry e;
+    if (cache_lookup(cache, key, e)) {
+      // validated like a hand-set schedule; a stale line falls through to tuning
+      if (sfk_op_set_schedule(op, &e.s) == SFK_OK) {
+        const bool cheby_ok = e.cheby[0] < 0 || (tp_ok(op) && e.cheby[0] < tp_shape_count(op->d.degree + 1) &&
+                                                 e.cheby[1] >= 0 && (e.cheby[2] == 0 || e.cheby[2] == 1));
+        if (cheby_ok) {
+          op->cheby_shape = e.cheby[0];
+          op->cheby_zc = e.cheby[1];
+          op->cheby_order = e.cheby[2];
+          if (best) *best = e.s;
+          if (best_seconds) *best_seconds = e.seconds;
+          return SFK_OK;
+        }
+      }
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
   const size_t bytes = size_t(op->nvec) * op->d.n_rhs * elem_size(op);
   void *x = nullptr, *y = nullptr;
@@ -306,6 +394,7 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
   op->cheby_shape = cwin[0];
   op->cheby_zc = cwin[1];
   op->cheby_order = cwin[2];
+  if (!key.empty()) cache_store(cache, key, CacheEntry{win, {cwin[0], cwin[1], cwin[2]}, win_t});
   op->sched_cb = win.cells_per_block;
   op->sched_order = win.block_order;
   op->cont_scatter = win.cont_scatter;

@@ -265,3 +265,33 @@ def test_cont_chebyshev_step(cuda, degree, scatter, c_tm):
     r_tm = (c_tm * tm if c_tm != 0.0 else 0.0) + 0.8 * diag * (au - rhs)
     assert rel_err(to_np(d_tm), r_tm) < TOL_FP64
     assert rel_err(to_np(d_new), u - r_tm) < TOL_FP64
+
+
+def test_tune_cache(cuda, tmp_path, monkeypatch):
+    """SFK_TUNE_CACHE: the first tune appends its result, a second operator
+    with the same key reads it back without timing (shown by editing the
+    stored line), a different mesh appends a new line"""
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("SFK_TUNE_CACHE", str(path))
+    nc, ij = (6, 5, 8), (6.0, 5.0, 8.0)
+    op = sfk.Operator(sfk.CONT_LAPLACE, 4, nc, inv_jac=ij)
+    sched, t = op.tune(repeats=2)
+    lines = path.read_text().splitlines()
+    assert len(lines) == 1
+    f = lines[0].split()
+    assert [int(v) for v in f[1:6]] == list(sched) and float(f[9]) == pytest.approx(t, rel=1e-6)
+    # a hand-edited line (another valid schedule, another time) is what the next tune returns
+    other = (0, 1 - sched[1], 2, (sched[3] + 1) % op.tile_shape(0)[0], 2)
+    f[1:6] = [str(v) for v in other]
+    f[9] = "0.125"
+    path.write_text(" ".join(f) + "\n")
+    op2 = sfk.Operator(sfk.CONT_LAPLACE, 4, nc, inv_jac=ij)
+    sched2, t2 = op2.tune(repeats=2)
+    assert tuple(sched2) == other and t2 == 0.125
+    assert op2.schedule[1:] == other[1:]
+    src = np.random.default_rng(4480).uniform(-1, 1, op2.vector_size)
+    assert rel_err(to_np(op2.apply(to_dev(src))), ob.cont_laplacian(4, nc, ij, src)) < TOL_FP64
+    # another mesh: a new line
+    op3 = sfk.Operator(sfk.CONT_LAPLACE, 4, (6, 5, 9), inv_jac=(6.0, 5.0, 9.0))
+    op3.tune(repeats=2)
+    assert len(path.read_text().splitlines()) == 2
