@@ -262,6 +262,15 @@ sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const double *b, 
  * diag_inv = D^{-1}.  DG-SIPG and the Laplacians. */
 sfk_status sfk_cheby_sweep(sfk_op *op, const void *rhs, void *x, void *spare, const void *diag_inv,
                            void *tmp, int degree, double lambda_min, double lambda_max, void *stream);
+/* No counterpart in the reference: the Chebyshev preconditioner of
+ * sfk_solve_cg_cheby, z = P(rhs) = the result of sfk_cheby_sweep from x = 0
+ * with the same degree and bounds, in the three-term form of the
+ * tensor-product kernel (four vector streams per step, D^{-1} formed in the
+ * kernel, the first iterate never stored; no diagonal vector needed).
+ * Continuous Laplacian with a constant coefficient (degree <= 8), n_rhs 1,
+ * degree >= 2; spare: a second work vector (distinct from z). */
+sfk_status sfk_cheby_precondition(sfk_op *op, const void *rhs, void *z, void *spare, int degree,
+                                  double lambda_min, double lambda_max, void *stream);
 
 /* ---- HIP graphs --------------------------------------------------------
  * Captures the sfk calls issued on a (non-default) stream between

@@ -109,6 +109,8 @@ def lib():
         L.sfk_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Schedule), _dp, _vp]
         L.sfk_cheby_sweep.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, _vp]
+        L.sfk_cheby_precondition.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             _vp]
         L.sfk_stream_create.argtypes = [ctypes.POINTER(_vp)]
         L.sfk_stream_destroy.argtypes = [_vp]
         L.sfk_graph_begin.argtypes = [_vp]
@@ -333,6 +335,15 @@ class Operator:
             self._check_vec(t)
         check(lib().sfk_cheby_sweep(self.handle, _dptr(rhs), _dptr(x), _dptr(spare), _dptr(diag_inv), _dptr(tmp),
                                     int(degree), float(lambda_min), float(lambda_max), _stream(stream)))
+
+    def cheby_precondition(self, rhs, degree, lambda_min, lambda_max, stream=None):
+        """z = P(rhs): the solver's Chebyshev preconditioner (three-term form on
+        the tensor-product kernel); equals cheby_sweep from x = 0"""
+        self._check_vec(rhs)
+        z, spare = self.new_vector(), self.new_vector()
+        check(lib().sfk_cheby_precondition(self.handle, _dptr(rhs), _dptr(z), _dptr(spare), int(degree),
+                                           float(lambda_min), float(lambda_max), _stream(stream)))
+        return z
 
     def grad_coefficient(self, u, lam, stream=None):
         import torch

@@ -149,6 +149,10 @@ struct TpArgs {
   T *tm;
   const T *prev;       // three-term Chebyshev: the previous iterate (or null: zero)
   T c_np, c_tm;
+  // three-term Chebyshev, CHEBY 3 / 4: the input / the previous iterate is
+  // the first iterate s_rhs D^{-1} rhs, formed in the kernel from rhs
+  // instead of read (CHEBY 3: src points to rhs)
+  T s_rhs;
   int nx, ny, nz;    // cells
   int gx, gy, gz;    // nodes
   int ntx, nty;      // tiles per direction
@@ -191,7 +195,8 @@ __device__ __forceinline__ void line_one(const void *tab, const T (&in)[2 * N - 
 // operator's own inverse diagonal (solver smoother): with tm = prev - u,
 // dst = u - c_tm (prev - u) - c_np D^{-1} (A u - rhs), four streams instead
 // of six, and dst may be prev (each node is read, then written, by one
-// thread)
+// thread); 3 and 4 the three-term step whose input (3) or previous iterate
+// (4) is the first iterate s_rhs D^{-1} rhs, formed from rhs in the kernel
 template <int N, typename T, int BX, int BY, int CHEBY>
 __global__ void __launch_bounds__((TpCfg<N, T, BX, BY>::THREADS))
 #if SFK_TP_WAVES
@@ -201,6 +206,9 @@ cont_tp_kernel(const TpArgs<T> a)
 {
   using C = TpCfg<N, T, BX, BY>;
   constexpr int p = N - 1, TX = C::TX, TY = C::TY, HX = C::HX, RPA = C::RPA, RPC = C::RPC;
+  // three-term form (2, 3, 4); 3: the input, 4: the previous iterate is
+  // s_rhs D^{-1} rhs (the first iterate, never stored)
+  constexpr bool TT = CHEBY >= 2, RSRC = CHEBY == 3, RPREV = CHEBY == 4;
   T *sA = dyn_lds<T>();
   T *sB = sA + C::LA;
   T *sC = sB + C::LA;
@@ -261,6 +269,20 @@ cont_tp_kernel(const TpArgs<T> a)
     } else {
 #pragma unroll
       for (int j = 0; j < 2 * N - 1; ++j) u[j] = T(0);
+    }
+    if constexpr (RSRC) {
+      {
+        // the line holds rhs: scale node j by s_rhs / D, D = a_xz dm[j % p] +
+        // b_xz dk_y[j % p] with the x/z factors of this line (row j of a
+        // line is node j % p of its cell; index 0 is the shared vertex);
+        // the Dirichlet nodes were loaded as 0
+        const auto &tb = tptab<N, T>(a.tab);
+        const int ix = xh % p, iz = (k0 + s) % p;
+        const T axz = tb.dk[0][ix] * tb.dm[iz] + tb.dm[ix] * tb.dk[2][iz];
+        const T bxz = tb.dm[ix] * tb.dm[iz];
+#pragma unroll
+        for (int j = 0; j < 2 * N - 1; ++j) u[j] *= a.s_rhs / (axz * tb.dm[j % p] + bxz * tb.dk[1][j % p]);
+      }
     }
   };
   auto store_y = [&]() {
@@ -323,13 +345,14 @@ cont_tp_kernel(const TpArgs<T> a)
   T cp = T(0), dp = T(0), yp = T(0);
 
   // output of node (column offset off) in plane Z (Dirichlet nodes only for
-  // CHEBY == 2: the inverse diagonal is 0 there)
+  // the three-term form: the inverse diagonal is 0 there)
   auto put = [&](int Z, unsigned off, T v) {
     const long long zo = (long long)Z * plane; // uniform
-    if constexpr (CHEBY == 2) {
+    if constexpr (TT) {
+      // (a first iterate formed from rhs is 0 on these nodes)
       const long long g = zo + off / (unsigned)sizeof(T);
-      const T us = src[g];
-      const T pv = a.prev ? a.prev[g] : T(0); // null: the zero vector
+      const T us = RSRC ? T(0) : src[g];
+      const T pv = !RSRC && !RPREV && a.prev ? a.prev[g] : T(0); // null: the zero vector
       dst[g] = us - a.c_tm * (pv - us);
     } else if constexpr (CHEBY == 1) {
       // r = A u - rhs; tm = c_tm tm + c_np diag r; dst = u - tm (five
@@ -360,12 +383,12 @@ cont_tp_kernel(const TpArgs<T> a)
       q_d[k] = a.diag[g];
       q_t[k] = a.c_tm != T(0) ? a.tm[g] : T(0);
       q_s[k] = src[g];
-    } else if constexpr (CHEBY == 2) {
+    } else if constexpr (TT) {
       // uniform plane bases + this lane's 32-bit column offset
       const long long zo = (long long)(ez * p + k) * plane;
       q_r[k] = at(a.rhs + zo, coff);
-      q_t[k] = a.prev ? at(a.prev + zo, coff) : T(0);
-      q_s[k] = at(src + zo, coff);
+      if constexpr (!RSRC && !RPREV) q_t[k] = a.prev ? at(a.prev + zo, coff) : T(0);
+      if constexpr (!RSRC) q_s[k] = at(src + zo, coff);
     }
   };
   if constexpr (CHEBY != 0) {
@@ -373,12 +396,12 @@ cont_tp_kernel(const TpArgs<T> a)
 #pragma unroll
       for (int k = 0; k < p; ++k) load_plane(ez_begin, k);
   }
-  // CHEBY == 2: the inverse diagonal of this node column in plane k of a
+  // three-term form: the inverse diagonal of this node column in plane k of a
   // cell, D = d1 dm[k] + d2 dk[2][k] with the x/y factors d1, d2 of the
   // column (0 off the interior columns); the same in every layer, so formed
   // once
-  T dinvk[CHEBY == 2 ? p : 1];
-  if constexpr (CHEBY == 2) {
+  T dinvk[TT ? p : 1];
+  if constexpr (TT) {
     const auto &tb = tptab<N, T>(a.tab);
     const bool inner = zone == 0 && X > 0 && Y > 0;
     const int ix = inner ? X % p : 0, iy = inner ? Y % p : 0;
@@ -389,11 +412,14 @@ cont_tp_kernel(const TpArgs<T> a)
   }
   // output v of this thread's node in plane ez p + k (prefetched streams)
   auto put_own = [&](int ez, int k, T v) {
-    if constexpr (CHEBY == 2) {
+    if constexpr (TT) {
       const long long zo = (long long)(ez * p + k) * plane; // uniform
       const T dinv = ez * p + k > 0 ? dinvk[k] : T(0);
-      const T t = a.c_tm * (q_t[k] - q_s[k]) + a.c_np * dinv * (v - q_r[k]);
-      at(dst + zo, coff) = q_s[k] - t;
+      const T z1 = a.s_rhs * dinv * q_r[k]; // the first iterate, when formed here
+      const T us = RSRC ? z1 : q_s[k];
+      const T pv = RPREV ? z1 : (RSRC ? T(0) : q_t[k]);
+      const T t = a.c_tm * (pv - us) + a.c_np * dinv * (v - q_r[k]);
+      at(dst + zo, coff) = us - t;
       if (ez + 1 < ez_end) load_plane(ez + 1, k);
     } else if constexpr (CHEBY == 1) {
       const long long g = (long long)(ez * p + k) * plane + coff / (unsigned)sizeof(T);
@@ -590,6 +616,10 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   a.diag = (const T *)L.diag;
   a.tm = (T *)L.tm;
   a.prev = (const T *)L.prev;
+  const bool rsrc = L.cheby == 2 && L.src_from_rhs, rprev = L.cheby == 2 && L.prev_from_rhs;
+  if (rsrc && rprev) return hipErrorInvalidValue;
+  if (rsrc) a.src = (const T *)L.rhs;
+  a.s_rhs = T(L.s_rhs);
   a.c_np = T(L.c_np);
   a.c_tm = T(L.c_tm);
   a.nx = L.nc[0];
@@ -610,7 +640,9 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   const long long blocks = (long long)a.ntx * a.nty * nchunks;
   if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
   const size_t lds = size_t(C::LDS_VALUES) * sizeof(T);
-  auto kern = L.cheby == 2 ? cont_tp_kernel<N, T, BX, BY, 2>
+  auto kern = rsrc ? cont_tp_kernel<N, T, BX, BY, 3>
+              : rprev ? cont_tp_kernel<N, T, BX, BY, 4>
+              : L.cheby == 2 ? cont_tp_kernel<N, T, BX, BY, 2>
                             : (L.cheby == 1 ? cont_tp_kernel<N, T, BX, BY, 1> : cont_tp_kernel<N, T, BX, BY, 0>);
   hipError_t e = allow_dynamic_lds((const void *)kern, lds);
   if (e != hipSuccess) return e;

@@ -59,6 +59,10 @@ struct TpLaunch {
   void *tm = nullptr;
   const void *prev = nullptr; // cheby 2: previous iterate (null: zero)
   double c_np = 0., c_tm = 0.;
+  // cheby 2: the input (src_from_rhs) or the previous iterate
+  // (prev_from_rhs) is s_rhs D^{-1} rhs, formed in the kernel (not read)
+  bool src_from_rhs = false, prev_from_rhs = false;
+  double s_rhs = 0.;
   int nc[3] = {1, 1, 1};
   int nrhs = 1;
   long long vstride = 0;

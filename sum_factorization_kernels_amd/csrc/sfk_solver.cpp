@@ -176,6 +176,42 @@ struct Solver {
     if (s_ != SFK_OK) return s_; \
   } while (0)
 
+// z = P(rr): the Chebyshev(degree >= 2) polynomial in D^{-1}A on [lmin,
+// lmax] applied to rr, in the three-term form on the tensor-product kernel:
+// z_1 = (1/theta) D^{-1} rr (z_0 = 0), z_{j+1} = z_j + c_tm (z_j - z_{j-1})
+// + c_np D^{-1}(rr - A z_j).  z_1 is never stored: the first step forms it
+// from rr as its input and the second as its previous iterate.  Step j
+// writes z_{j+1} into bufs[(j + 1) % 2] (z_2 into *z_io, z_3 into *spare,
+// then each over z_{j-1}: the kernel allows dst == prev); the result ends
+// in *z_io (the two pointers are swapped when it lands in the other buffer).
+hipError_t cheby_tp3(sfk_op *sm, const void *rr, void **z_io, void **spare, int degree, double lmin, double lmax,
+                     hipStream_t s)
+{
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma1 = theta / delta;
+  void *bufs[2] = {*z_io, *spare};
+  double rho_old = 1. / sigma1;
+  for (int j = 1; j < degree; ++j) {
+    const double rho = 1. / (2. * sigma1 - rho_old);
+    TpLaunch T;
+    fill_tp_launch(sm, T, true);
+    T.cheby = 2;
+    T.rhs = rr;
+    T.s_rhs = 1. / theta;
+    T.src = j == 1 ? rr : bufs[j % 2];
+    T.src_from_rhs = j == 1;
+    T.prev = j == 1 ? nullptr : (j == 2 ? rr : bufs[(j + 1) % 2]); // j == 1: the zero z_0
+    T.prev_from_rhs = j == 2;
+    T.dst = bufs[(j + 1) % 2];
+    T.c_np = 2. * rho / delta;
+    T.c_tm = rho * rho_old;
+    const hipError_t e = launch_cont_tp(T, s);
+    if (e != hipSuccess) return e;
+    rho_old = rho;
+  }
+  if (degree % 2 == 1) std::swap(*z_io, *spare); // z_k is in bufs[k % 2]
+  return hipSuccess;
+}
+
 } // namespace
 } // namespace sfk
 
@@ -281,32 +317,12 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   // buffers, the result ends in *z_io (pointers swapped as needed)
   auto cheby = [&](const void *rr, void **z_io, void **spare) -> sfk_status {
     if (tp3) {
-      // z_1 = (1/theta) D^{-1} rr (z_0 = 0); z_{j+1} = z_j + c_tm (z_j -
-      // z_{j-1}) + c_np D^{-1}(rr - A z_j), written over z_{j-1}
-      void *zc = *z_io, *other = *spare;
-      SOLVER_HIP(vec_mul_scaled(f32, n, 1. / theta, inv_sm, rr, zc, s));
-      const void *prev = nullptr;
-      double rho_old = 1. / sigma1;
-      for (int j = 1; j < cheby_degree; ++j) {
-        const double rho = 1. / (2. * sigma1 - rho_old);
-        void *out = prev ? const_cast<void *>(prev) : other;
-        TpLaunch T;
-        fill_tp_launch(S.sm, T, true);
-        T.cheby = 2;
-        T.src = zc;
-        T.prev = prev;
-        T.rhs = rr;
-        T.dst = out;
-        T.c_np = 2. * rho / delta;
-        T.c_tm = rho * rho_old;
-        SOLVER_HIP(launch_cont_tp(T, s));
-        prev = zc;
-        zc = out;
-        rho_old = rho;
-        ++S.st.matvecs_inner;
+      if (cheby_degree == 1) {
+        SOLVER_HIP(vec_mul_scaled(f32, n, 1. / theta, inv_sm, rr, *z_io, s));
+        return SFK_OK;
       }
-      *spare = zc == *z_io ? *spare : *z_io;
-      *z_io = zc;
+      SOLVER_HIP(cheby_tp3(S.sm, rr, z_io, spare, cheby_degree, lmin, lmax, s));
+      S.st.matvecs_inner += cheby_degree - 1;
       return SFK_OK;
     }
     void *z = *z_io;
@@ -442,6 +458,37 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
              S.st.iterations, S.st.final_residual / bnorm);
     set_error(msg);
     return SFK_ERR_NO_CONVERGENCE;
+  }
+  return SFK_OK;
+}
+
+extern "C" sfk_status sfk_cheby_precondition(sfk_op *op, const void *rhs, void *z, void *spare, int degree,
+                                             double lambda_min, double lambda_max, void *stream)
+{
+  if (!op || !rhs || !z || !spare || z == spare || degree < 2 || degree > 32 || !(lambda_min > 0) ||
+      !(lambda_max > lambda_min)) {
+    set_error("sfk_cheby_precondition: invalid argument");
+    return SFK_ERR_INVALID_ARG;
+  }
+  if (!uses_tp(op)) {
+    set_error("sfk_cheby_precondition: continuous Laplacian on the tensor-product kernel only (constant "
+              "coefficient, degree <= 8)");
+    return SFK_ERR_UNSUPPORTED;
+  }
+  if (op->d.n_rhs != 1) {
+    set_error("sfk_cheby_precondition: n_rhs must be 1");
+    return SFK_ERR_UNSUPPORTED;
+  }
+  sfk_status st = ensure_device_tables(op);
+  if (st != SFK_OK) return st;
+  void *zz = z, *sp = spare;
+  hipError_t e = cheby_tp3(op, rhs, &zz, &sp, degree, lambda_min, lambda_max, (hipStream_t)stream);
+  // odd degree: the result is in the spare vector; copy it to z
+  if (e == hipSuccess && zz != z)
+    e = hipMemcpyAsync(z, zz, size_t(op->nvec) * elem_size(op), hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_error(std::string("sfk_cheby_precondition: ") + hipGetErrorString(e));
+    return SFK_ERR_HIP;
   }
   return SFK_OK;
 }

@@ -295,3 +295,30 @@ def test_tune_cache(cuda, tmp_path, monkeypatch):
     op3 = sfk.Operator(sfk.CONT_LAPLACE, 4, (6, 5, 9), inv_jac=(6.0, 5.0, 9.0))
     op3.tune(repeats=2)
     assert len(path.read_text().splitlines()) == 2
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("degree", [2, 3, 6])
+@pytest.mark.parametrize("p", [2, 4, 6])
+def test_cheby_precondition_matches_sweep(cuda, prec, degree, p):
+    """the solver's preconditioner (three-term Chebyshev on the tensor-product
+    kernel, first iterate formed from rhs in the kernel) equals the tm-form
+    sweep from x = 0 (a separate kernel path) with the same bounds, on every
+    node, for NaN-prefilled work vectors"""
+    import torch
+    nc, ij = (5, 4, 6), (5.0, 4.0, 6.0)
+    pr = sfk.FP32 if prec == "fp32" else sfk.FP64
+    dt = torch.float32 if prec == "fp32" else torch.float64
+    op = sfk.Operator(sfk.CONT_LAPLACE, p, nc, inv_jac=ij, precision=pr)
+    op.set_schedule(0, 1, 2, 1, 2)
+    N = op.vector_size
+    rng = np.random.default_rng(4490 + p + degree)
+    rhs = to_dev(rng.standard_normal(N), dt) * (op.diagonal() != 0)  # Dirichlet rows of a residual are 0
+    d = op.diagonal()
+    dinv = torch.where(d != 0, 1.0 / d, torch.zeros_like(d))
+    lmax, lmin = 2.5, 2.5 / 30.0  # (bounds of the order of lambda(D^{-1}A); the two paths share them)
+    z = op.cheby_precondition(rhs, degree, lmin, lmax)
+    x, spare, tm = torch.zeros_like(rhs), _nan_vec(op), _nan_vec(op)
+    op.cheby_sweep(rhs, x, spare, dinv, tm, degree, lmin, lmax)
+    tol = 5e-5 if prec == "fp32" else 1e-10
+    assert rel_err(to_np(z), to_np(x)) < tol
