@@ -39,6 +39,8 @@
 // stores).  Optionally the Chebyshev update of do_cheby_iter<true>
 // (evaluation_dg_laplacian.h:485-502) is fused into phase Z: the assembled
 // value of a node is final there, and all its streams are coalesced.
+#include <type_traits>
+
 #include "sfk_device.h"
 #include "sfk_kernels.h"
 
@@ -121,7 +123,7 @@ constexpr int tp_waves()
 template <int N, typename T, int BX, int BY, int CHEBY>
 constexpr int tp_waves_of()
 {
-  return CHEBY == 0 ? tp_waves<N, T, BX, BY>()
+  return CHEBY == 0 || CHEBY == 5 ? tp_waves<N, T, BX, BY>()
                     : (SFK_TP_CHEBY_WAVES > 0 ? SFK_TP_CHEBY_WAVES
                                               : (SFK_TP_CHEBY_WAVES < 0 ? tp_waves<N, T, BX, BY>() : 1));
 }
@@ -153,6 +155,7 @@ struct TpArgs {
   // the first iterate s_rhs D^{-1} rhs, formed in the kernel from rhs
   // instead of read (CHEBY 3: src points to rhs)
   T s_rhs;
+  double *dot_part;  // CHEBY 5: per-workgroup partial sums of src . dst
   int nx, ny, nz;    // cells
   int gx, gy, gz;    // nodes
   int ntx, nty;      // tiles per direction
@@ -208,7 +211,7 @@ cont_tp_kernel(const TpArgs<T> a)
   constexpr int p = N - 1, TX = C::TX, TY = C::TY, HX = C::HX, RPA = C::RPA, RPC = C::RPC;
   // three-term form (2, 3, 4); 3: the input, 4: the previous iterate is
   // s_rhs D^{-1} rhs (the first iterate, never stored)
-  constexpr bool TT = CHEBY >= 2, RSRC = CHEBY == 3, RPREV = CHEBY == 4;
+  constexpr bool TT = CHEBY >= 2 && CHEBY <= 4, RSRC = CHEBY == 3, RPREV = CHEBY == 4, DOT = CHEBY == 5;
   T *sA = dyn_lds<T>();
   T *sB = sA + C::LA;
   T *sC = sB + C::LA;
@@ -374,8 +377,9 @@ cont_tp_kernel(const TpArgs<T> a)
   // registers of plane k take the loads of layer ez + 1, in flight during the
   // rest of phase Z and the next layer's phases Y and X (loaded at the start
   // of their own layer, their latency was only partly hidden)
-  constexpr int NCH = CHEBY ? p : 1;
+  constexpr int NCH = CHEBY ? p : 1; // CHEBY 5: q_s only
   T q_r[NCH], q_d[CHEBY == 1 ? NCH : 1], q_t[NCH], q_s[NCH];
+  double dot_acc = 0.; // CHEBY 5
   auto load_plane = [&](int ez, int k) {
     if constexpr (CHEBY == 1) {
       const long long g = (long long)(ez * p + k) * plane + coff / (unsigned)sizeof(T);
@@ -383,6 +387,8 @@ cont_tp_kernel(const TpArgs<T> a)
       q_d[k] = a.diag[g];
       q_t[k] = a.c_tm != T(0) ? a.tm[g] : T(0);
       q_s[k] = src[g];
+    } else if constexpr (DOT) {
+      q_s[k] = at(src + (long long)(ez * p + k) * plane, coff);
     } else if constexpr (TT) {
       // uniform plane bases + this lane's 32-bit column offset
       const long long zo = (long long)(ez * p + k) * plane;
@@ -430,6 +436,10 @@ cont_tp_kernel(const TpArgs<T> a)
       if (ez + 1 < ez_end) load_plane(ez + 1, k);
     } else {
       put(ez * p + k, coff, v);
+      if constexpr (DOT) {
+        dot_acc += double(q_s[k]) * double(v);
+        if (ez + 1 < ez_end) load_plane(ez + 1, k);
+      }
     }
   };
   // zero (Dirichlet) outputs of plane Z for this thread: its own node when
@@ -515,6 +525,23 @@ cont_tp_kernel(const TpArgs<T> a)
     const int Z = gz - 1;
     if (zone == 0) put(Z, coff, T(0));
     put_zeros(Z, 0, -1);
+  }
+  if constexpr (DOT) {
+    // workgroup sum of this thread's src . dst terms: within each wave by
+    // lane exchanges, then over the waves through LDS (free after the march)
+    double v = dot_acc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(sA);
+    const int wave = threadIdx.x / 64, nw = (C::THREADS + 63) / 64;
+    if (threadIdx.x % 64 == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.;
+      for (int w = 0; w < nw; ++w) t += red[w];
+      a.dot_part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
   }
 }
 
@@ -619,6 +646,7 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   const bool rsrc = L.cheby == 2 && L.src_from_rhs, rprev = L.cheby == 2 && L.prev_from_rhs;
   if (rsrc && rprev) return hipErrorInvalidValue;
   if (rsrc) a.src = (const T *)L.rhs;
+  a.dot_part = L.dot_part;
   a.s_rhs = T(L.s_rhs);
   a.c_np = T(L.c_np);
   a.c_tm = T(L.c_tm);
@@ -640,8 +668,14 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   const long long blocks = (long long)a.ntx * a.nty * nchunks;
   if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
   const size_t lds = size_t(C::LDS_VALUES) * sizeof(T);
+  // the matvec with src . dst (CHEBY 5) exists for FP64 only (the solver's
+  // outer matvec)
+  decltype(&cont_tp_kernel<N, T, BX, BY, 0>) kdot = nullptr;
+  if constexpr (std::is_same<T, double>::value) kdot = cont_tp_kernel<N, T, BX, BY, 5>;
+  if (L.dot_part && (L.cheby != 0 || L.nrhs != 1 || !kdot)) return hipErrorInvalidValue;
   auto kern = rsrc ? cont_tp_kernel<N, T, BX, BY, 3>
               : rprev ? cont_tp_kernel<N, T, BX, BY, 4>
+              : L.dot_part ? kdot
               : L.cheby == 2 ? cont_tp_kernel<N, T, BX, BY, 2>
                             : (L.cheby == 1 ? cont_tp_kernel<N, T, BX, BY, 1> : cont_tp_kernel<N, T, BX, BY, 0>);
   hipError_t e = allow_dynamic_lds((const void *)kern, lds);
@@ -696,6 +730,15 @@ static hipError_t launch_tp_t(const TpLaunch &L, hipStream_t s)
   SFK_TP_SHAPES(SFK_X)
 #undef SFK_X
   return hipErrorInvalidValue;
+}
+
+long long tp_launch_blocks(const TpLaunch &L)
+{
+  int bx = 0, by = 0;
+  tp_shape(L.n, L.shape, &bx, &by);
+  if (bx <= 0 || by <= 0) return 0;
+  const int nz = L.nc[2], zc = L.zc > 0 ? (L.zc < nz ? L.zc : nz) : nz;
+  return (long long)((L.nc[0] + bx - 1) / bx) * ((L.nc[1] + by - 1) / by) * ((nz + zc - 1) / zc);
 }
 
 hipError_t launch_cont_tp(const TpLaunch &L, hipStream_t s)

@@ -63,6 +63,9 @@ struct TpLaunch {
   // (prev_from_rhs) is s_rhs D^{-1} rhs, formed in the kernel (not read)
   bool src_from_rhs = false, prev_from_rhs = false;
   double s_rhs = 0.;
+  // cheby 0, FP64, one right-hand side: also write the workgroups' partial
+  // sums of src . dst here (tp_launch_blocks of them)
+  double *dot_part = nullptr;
   int nc[3] = {1, 1, 1};
   int nrhs = 1;
   long long vstride = 0;
@@ -71,6 +74,7 @@ struct TpLaunch {
   int order = 1;        // block order
 };
 hipError_t launch_cont_tp(const TpLaunch &L, hipStream_t stream);
+long long tp_launch_blocks(const TpLaunch &L); // workgroups of one right-hand side
 hipError_t upload_tp_tables(const Tables1D &t, const double inv_jac[3], bool fp32, void **dev);
 int tp_shape_count(int n);
 void tp_shape(int n, int shape, int *bx, int *by);
@@ -172,6 +176,8 @@ hipError_t vec_dot(bool fp32, const void *x, const void *y, long long n, double 
 hipError_t upload_device_tables(const Tables1D &t, bool fp32, void **dev);
 
 // result_dev[0] = x.y, result_dev[1] = x.w (FP64 vectors; partials: 2*n_partials)
+// result_dev[0] = sum of count partial sums (of a fused kernel)
+hipError_t vec_sum_partials(const double *partials, long long count, double *result_dev, hipStream_t s);
 hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, double *result_dev, double *partials,
                     int n_partials, hipStream_t s);
 // solver kernels (FP64 vectors): x += alpha p, r -= alpha q, optionally

@@ -246,9 +246,18 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   // device work space: FP64 vectors, and smoother-precision vectors
   DevBuf red, d64, inv64, isq, r, w, lv0, lv1, lw, lt, e64;
   DevBuf rs, zs, qs, tms, inv_s;
-  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (2 * kPartials + 2)));
+  // the outer matvec on the tensor-product kernel also sums p . q (its
+  // workgroups' partial sums, one per workgroup)
+  const bool fused_pq = uses_tp(op) && op->d.n_rhs == 1;
+  long long n_red = 2 * kPartials;
+  if (fused_pq) {
+    TpLaunch T;
+    fill_tp_launch(op, T);
+    if (tp_launch_blocks(T) > n_red) n_red = tp_launch_blocks(T);
+  }
+  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (n_red + 2)));
   S.partials = (double *)red.p;
-  S.dres = S.partials + 2 * kPartials; // after the partials of two sums: two result slots
+  S.dres = S.partials + n_red; // after the partial sums: two result slots
   for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
     SOLVER_HIP(hipMalloc(&bb->p, n8));
   for (DevBuf *bb : {&rs, &zs, &qs, &tms, &inv_s})
@@ -404,10 +413,22 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
     double rz;
     SOLVER_ST(S.dot(false, r.p, z64, rz));
     while (iters < max_iter) {
-      SOLVER_ST(sfk_apply(op, pv, qv, stream));
-      ++S.st.matvecs_outer;
       double pq;
-      SOLVER_ST(S.dot(false, pv, qv, pq));
+      if (fused_pq) {
+        SOLVER_ST(ensure_device_tables(op));
+        TpLaunch T;
+        fill_tp_launch(op, T);
+        T.src = pv;
+        T.dst = qv;
+        T.dot_part = S.partials;
+        SOLVER_HIP(launch_cont_tp(T, s));
+        SOLVER_HIP(vec_sum_partials(S.partials, tp_launch_blocks(T), S.dres, s));
+        SOLVER_ST(S.results(&pq, 1));
+      } else {
+        SOLVER_ST(sfk_apply(op, pv, qv, stream));
+        SOLVER_ST(S.dot(false, pv, qv, pq));
+      }
+      ++S.st.matvecs_outer;
       if (!(pq > 0.)) {
         set_error("sfk_solve_cg_cheby: operator not positive definite on the search direction");
         return SFK_ERR_NONFINITE;

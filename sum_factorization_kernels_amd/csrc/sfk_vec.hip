@@ -255,6 +255,13 @@ hipError_t vec_dot(bool fp32, const void *x, const void *y, long long n, double 
   return hipGetLastError();
 }
 
+hipError_t vec_sum_partials(const double *partials, long long count, double *result_dev, hipStream_t s)
+{
+  if (count < 0 || count > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, partials, (int)count, result_dev);
+  return hipGetLastError();
+}
+
 hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, double *result_dev, double *partials,
                     int n_partials, hipStream_t s)
 {
