@@ -380,7 +380,7 @@ def extras(args, configs):
     configs["c1_q4_16cube"] = c1
     del g1, st1, o1, x1, y1
     # c1 scaled to 128^3 cells (HBM regime)
-    tb, ob, schedb, _, _ = run_cont(4, 128, sfk.FP64, 20, 3)
+    tb, ob, schedb, _, _ = run_cont(4, 128, sfk.FP64, args.steps, args.warmup)
     configs["c1_scaled_q4_128cube"] = cont_entry("Q4 128^3 FP64", 4, 128, sfk.FP64, tb, schedb,
                                                  "cont_tp_q4_128cube_fp64_pmc.txt")
     del ob
