@@ -281,12 +281,12 @@ def main():
                                                  "cont_tp_q6_64cube_fp32_pmc.txt")
     configs["c3_q6_64cube"]["fp32_over_fp64_matvec_speedup"] = round(t64 / t32, 3)
     # CG-Chebyshev to 1e-10: both smoothers at degree 6 (the configuration)
-    # and each at its best degree of {4, 6, 8}
+    # and each at its best degree of {4, 6, 8, 12, 16}
     b3 = o3.rhs_constant(1.0)
     bn = float(torch.linalg.norm(b3))
     try:
         runs = {}
-        for deg in ((6,) if args.no_extras else (4, 6, 8)):
+        for deg in ((6,) if args.no_extras else (4, 6, 8, 12, 16)):
             for name, sm in (("mixed", s32), ("fp64", o3)):
                 x3.zero_()
                 runs[(name, deg)] = sfk.solve_cg_cheby(o3, sm, b3, x3, 1e-10, 3000, deg, 12)
