@@ -488,8 +488,9 @@ def extras(args, configs):
         x6 = torch.randn(o6.vector_size, dtype=dt, device="cuda")
         y6 = torch.empty_like(x6)
         t6 = timed(lambda: o6.apply(x6, y6), args.steps, args.warmup)
+        tr6 = read_traffic("precomputed_dg_p3_56x50x55_fp64.txt") if name == "fp64" else None
         pc[name] = {"gdofs": round(o6.vector_size / t6 / 1e9, 3), "apply_us": round(t6 * 1e6, 2),
-                    "schedule": list(sched6), "roofline": roofline(2 * x6.element_size() * o6.vector_size, t6)}
+                    "schedule": list(sched6), "roofline": roofline(2 * x6.element_size() * o6.vector_size, t6, tr6)}
         log(f"precomputed DG p3 56x50x55 {name}: {o6.vector_size / t6 / 1e9:.2f} GDoF/s")
         del o6, x6, y6
     configs["precomputed_dg_p3_56x50x55"] = pc
