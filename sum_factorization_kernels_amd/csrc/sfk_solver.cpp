@@ -317,10 +317,7 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   // the smoother's Chebyshev steps in three-term form on the tensor-product
   // kernel (four vector streams per step instead of six, the inverse
   // diagonal formed in the kernel) when the smoother operator has it
-#ifndef SFK_SOLVER_TP3
-#define SFK_SOLVER_TP3 1
-#endif
-  const bool tp3 = SFK_SOLVER_TP3 && uses_tp(S.sm);
+  const bool tp3 = uses_tp(S.sm);
   if (tp3) SOLVER_ST(ensure_device_tables(S.sm));
   // z = P(rr) on the smoother operator; *z_io and *spare are two distinct
   // buffers, the result ends in *z_io (pointers swapped as needed)
