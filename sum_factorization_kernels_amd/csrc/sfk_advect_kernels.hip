@@ -276,6 +276,10 @@ advect_kernel(const AdvArgs<N, T> a)
       face_sweep_t<N, T>(tb.Sf, F, 6, l, false);
     }
     __syncthreads();
+    // cell gradients of line l in direction DIR: the x and y lines go
+    // through LDS below, so that every store writes consecutive dofs across
+    // lanes (FP64 p3 48^3: 168 -> 151 us)
+    T gx[N], gy[N];
     if (act) {
       const auto &tb = ctab(a.dt);
       auto do_dir = [&](auto dirc) {
@@ -286,9 +290,10 @@ advect_kernel(const AdvArgs<N, T> a)
         eo_apply<N, -1>(tb.Df, mq, g);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          const int dof = line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>();
-          a.dst[(cell * 3 + DIR) * N3 + dof] =
-              g[i] * a.ij[DIR] * uq[i] * (a.jxw * line_weight<N, DIR>(tb.w, l, i));
+          const T gi = g[i] * a.ij[DIR] * uq[i] * (a.jxw * line_weight<N, DIR>(tb.w, l, i));
+          if constexpr (DIR == 0) gx[i] = gi;
+          else if constexpr (DIR == 1) gy[i] = gi;
+          else a.dst[(cell * 3 + DIR) * N3 + line_base<N, N, DIR>(l) + i * line_stride<N, N, DIR>()] = gi;
         }
         T um = 0, mm = 0;
 #pragma unroll
@@ -306,6 +311,23 @@ advect_kernel(const AdvArgs<N, T> a)
       do_dir(std::integral_constant<int, 0>());
       do_dir(std::integral_constant<int, 1>());
       do_dir(std::integral_constant<int, 2>());
+    }
+    // x and y gradients: written into the cell blocks (their q-point values
+    // are no longer needed) and stored as z lines, lane l at dof l + m N^2
+    __syncthreads();
+    if (act) {
+      cl_write<N, 0>(A, l, gx);
+      cl_write<N, 1>(B, l, gy);
+    }
+    __syncthreads();
+    if (act) {
+      T v[N];
+      cl_read<N, 2>(A, l, v);
+#pragma unroll
+      for (int m = 0; m < N; ++m) a.dst[(cell * 3 + 0) * N3 + l + m * N2] = v[m];
+      cl_read<N, 2>(B, l, v);
+#pragma unroll
+      for (int m = 0; m < N; ++m) a.dst[(cell * 3 + 1) * N3 + l + m * N2] = v[m];
     }
     return;
   } else {
