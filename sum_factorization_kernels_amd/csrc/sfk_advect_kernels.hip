@@ -410,21 +410,30 @@ advect_kernel(const AdvArgs<N, T> a)
       cl_read<N, 0>(A, l, u);
       if constexpr (!ADJ) eo_apply<N, 1>(tb.If, u, o);
       else eo_apply<N, 1>(tb.Sb, u, o);
-      const long long g0 = cell * N3 + l * N;
+      cl_write<N, 0>(B, l, o);
+    }
+    // the x lines go back through LDS and leave as z lines: lane l holds
+    // dofs l + m N^2, so every store writes consecutive dofs across lanes
+    __syncthreads();
+    if (act) {
+      T o[N];
+      cl_read<N, 2>(B, l, o);
+      const long long g0 = cell * N3 + l;
       if constexpr (STAGE) {
         // evaluation_dg_advect.h:651-667
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          const T vnp = a.upd[g0 + i] + a.c_np * o[i];
-          a.dst[g0 + i] = vnp;
-          if (a.c_tm != T(0)) a.tm[g0 + i] = vnp + a.c_tm * o[i];
+        for (int m = 0; m < N; ++m) {
+          const long long g = g0 + m * N2;
+          const T vnp = a.upd[g] + a.c_np * o[m];
+          a.dst[g] = vnp;
+          if (a.c_tm != T(0)) a.tm[g] = vnp + a.c_tm * o[m];
         }
       } else {
         // written once, read by nobody in this launch: non-temporal stores
         // keep L2 for the velocity and neighbour layers other workgroups
         // re-read (forward p = 3, 48^3: 73 -> 60 us on MI355X)
 #pragma unroll
-        for (int i = 0; i < N; ++i) __builtin_nontemporal_store(o[i], a.dst + g0 + i);
+        for (int m = 0; m < N; ++m) __builtin_nontemporal_store(o[m], a.dst + g0 + m * N2);
       }
     }
   }
