@@ -384,18 +384,19 @@ sipg_general_kernel(const GenArgs<N, T> a)
   }
   __syncthreads();
   // ----------------------------------------------------------------- Sy
-  // Chebyshev: this thread's x line of rhs, diag, tmp and the input, loaded
-  // one phase ahead of the update that uses them
+  // Chebyshev: rhs, diag, tmp and the input at the dofs this thread stores
+  // (its z line, dofs l + m N^2), loaded two phases ahead of the update
   T crhs[CHEBY ? N : 1], cdiag[CHEBY ? N : 1], ctm[CHEBY ? N : 1], csrc[CHEBY ? N : 1];
   if (act) {
     if constexpr (CHEBY) {
-      const long long g0 = cell * N3 + l * N;
+      const long long g0 = cell * N3 + l;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        crhs[i] = a.rhs[g0 + i];
-        cdiag[i] = a.diag[g0 + i];
-        ctm[i] = a.c_tm != T(0) ? a.tm[g0 + i] : T(0); // c_tm == 0: tm not read
-        csrc[i] = a.src[g0 + i];
+        const long long g = g0 + i * N2;
+        crhs[i] = a.rhs[g];
+        cdiag[i] = a.diag[g];
+        ctm[i] = a.c_tm != T(0) ? a.tm[g] : T(0); // c_tm == 0: tm not read
+        csrc[i] = a.src[g];
       }
     }
     const auto &tb = ctab(a.dt);
@@ -413,24 +414,36 @@ sipg_general_kernel(const GenArgs<N, T> a)
     eo_apply<N, 1>(tb.Sb, u, o);
     if constexpr (DIAG) {
       if (unit / N == l) a.dst[cell * N3 + unit] = o[unit % N];
+    } else if constexpr (CHEBY) {
+      lds_write_line<N, N, 0>(Q, l, o); // this thread's own line: no race
     } else {
+      // written once, read by nobody in this launch: non-temporal stores
+      // keep L2 for the neighbour layers and face metrics other workgroups
+      // re-read (measured 3-4 % faster on MI355X).  Staged through LDS as
+      // z lines like the Chebyshev step below, the extra barrier cost more
+      // (150 -> 158 us) than the coalescing saved.
       const long long g0 = cell * N3 + l * N;
-      if constexpr (CHEBY) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          const long long g = g0 + i;
-          const T res = o[i] - crhs[i];
-          const T base = a.c_np * cdiag[i] * res;
-          const T tmv = a.c_tm != T(0) ? a.c_tm * ctm[i] + base : base;
-          a.tm[g] = tmv;
-          a.dst[g] = csrc[i] - tmv;
-        }
-      } else {
-        // written once, read by nobody in this launch: non-temporal stores
-        // keep L2 for the neighbour layers and face metrics other
-        // workgroups re-read (measured 3-4 % faster on MI355X)
+      for (int i = 0; i < N; ++i) __builtin_nontemporal_store(o[i], a.dst + g0 + i);
+    }
+  }
+  if constexpr (CHEBY && !DIAG) {
+    // the Chebyshev step's six streams (rhs, diag, tmp, src in; tmp, dst
+    // out) move as z lines (dofs l + m N^2): every access is consecutive
+    // across lanes (the Chebyshev(5) sweep at p5 32^3: 1.135 -> 1.04 ms)
+    __syncthreads();
+    if (act) {
+      T o[N];
+      lds_read_line<N, N, 2>(Q, l, o);
+      const long long g0 = cell * N3 + l;
 #pragma unroll
-        for (int i = 0; i < N; ++i) __builtin_nontemporal_store(o[i], a.dst + g0 + i);
+      for (int i = 0; i < N; ++i) {
+        const long long g = g0 + i * N2;
+        const T res = o[i] - crhs[i];
+        const T base = a.c_np * cdiag[i] * res;
+        const T tmv = a.c_tm != T(0) ? a.c_tm * ctm[i] + base : base;
+        a.tm[g] = tmv;
+        a.dst[g] = csrc[i] - tmv;
       }
     }
   }
