@@ -53,6 +53,16 @@ def test_sipg_p5_32cube_deformed(cuda):
     op32 = sfk.Operator(sfk.DG_SIPG, degree, nc, geometry=sfk.GEOM_PER_Q, precision=sfk.FP32)
     op32.set_sine_deformation(0.1)
     assert rel_err(to_np(op32.apply(to_dev(x, torch.float32))), ref) < TOL_FP32
+    # the schedules a benchmark runs: the autotuner's pick on this device
+    # and the cells-per-block / block order of the recorded runs (3, 2)
+    for sched in ("tuned", (3, 2, -1)):
+        for o in (op, op32):
+            if sched == "tuned":
+                o.tune(repeats=2)
+            else:
+                o.set_schedule(*sched)
+        assert rel_err(to_np(op.apply(to_dev(x))), ref) < TOL_FP64, sched
+        assert rel_err(to_np(op32.apply(to_dev(x, torch.float32))), ref) < TOL_FP32, sched
     # Chebyshev(5) sweep from x with rhs = 1 on [lmax/30, lmax]
     dinv = 1.0 / op.diagonal()
     rhs = np.ones(op.vector_size)
@@ -89,6 +99,12 @@ def test_q6_64cube_matvec_and_solves(cuda):
     ref = ob.cont_laplacian(6, nc, ij, x)
     assert rel_err(to_np(op.apply(to_dev(x))), ref) < TOL_FP64
     assert rel_err(to_np(s32.apply(to_dev(x, torch.float32))), ref) < TOL_FP32
+    # the autotuner's schedules (tile shape, z cells, block order: what a
+    # benchmark runs), then the solves on them
+    op.tune(repeats=2)
+    s32.tune(repeats=2)
+    assert rel_err(to_np(op.apply(to_dev(x))), ref) < TOL_FP64, op.schedule
+    assert rel_err(to_np(s32.apply(to_dev(x, torch.float32))), ref) < TOL_FP32, s32.schedule
     del ref, x
     b = op.rhs_constant(1.0)
     bn = float(torch.linalg.norm(b))
@@ -114,10 +130,18 @@ def test_varcoef_q8_24cube_8rhs(cuda):
     N = op.vector_size
     rng = np.random.default_rng(6)
     u = rng.standard_normal(8 * N)
-    out = to_np(op.apply(to_dev(u, torch.float32)))
-    for r in (0, 3, 7):
-        ref = ob.cont_laplacian(degree, nc, ij, u[r * N:(r + 1) * N].copy(), coef=k)
-        assert rel_err(out[r * N:(r + 1) * N], ref) < TOL_FP32, r
+    refs = {r: ob.cont_laplacian(degree, nc, ij, u[r * N:(r + 1) * N].copy(), coef=k) for r in (0, 3, 7)}
+    # the default schedule, the autotuner's pick on this device, and the
+    # schedule of the recorded runs (3 cells per workgroup, block order 1,
+    # colored stores)
+    for sched in ("default", "tuned", (3, 1, 1)):
+        if sched == "tuned":
+            op.tune(repeats=2)
+        elif sched != "default":
+            op.set_schedule(*sched)
+        out = to_np(op.apply(to_dev(u, torch.float32)))
+        for r, ref in refs.items():
+            assert rel_err(out[r * N:(r + 1) * N], ref) < TOL_FP32, (sched, r)
     lam = rng.standard_normal(8 * N)
     g = to_np(op.grad_coefficient(to_dev(u, torch.float32), to_dev(lam, torch.float32)))
     gref = ob.cont_laplacian_coef_grad(degree, nc, ij, u, lam, n_rhs=8)
