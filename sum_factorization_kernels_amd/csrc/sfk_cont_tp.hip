@@ -212,6 +212,10 @@ cont_tp_kernel(const TpArgs<T> a)
   // three-term form (2, 3, 4); 3: the input, 4: the previous iterate is
   // s_rhs D^{-1} rhs (the first iterate, never stored)
   constexpr bool TT = CHEBY >= 2 && CHEBY <= 4, RSRC = CHEBY == 3, RPREV = CHEBY == 4, DOT = CHEBY == 5;
+  // CHEBY 5: a layer's outputs held in registers and stored, with the
+  // prefetch of the next layer's input at the node, after the next layer's
+  // wait for its y lines (see flush below)
+  constexpr bool DELAY = DOT;
   T *sA = dyn_lds<T>();
   T *sB = sA + C::LA;
   T *sC = sB + C::LA;
@@ -454,6 +458,8 @@ cont_tp_kernel(const TpArgs<T> a)
     if (xtra_x && xtra_y) put(Z, coff + (unsigned)((gx + 1) * (int)sizeof(T)), T(0));
   };
 
+  // delayed outputs: po[k] of plane ez p + k of the last layer phase Z stored
+  T po[DELAY ? N - 1 : 1];
   auto phase_z = [&](int ez, bool store) {
     if (zone == 0) {
       T c[N], d[N], o[N];
@@ -478,16 +484,39 @@ cont_tp_kernel(const TpArgs<T> a)
 #pragma unroll
         for (int k = 0; k < p; ++k) {
           const int Z = ez * p + k;
-          put_own(ez, k, (bxy || Z == 0) ? T(0) : o[k]);
+          if constexpr (DELAY) po[k] = (bxy || Z == 0) ? T(0) : o[k];
+          else put_own(ez, k, (bxy || Z == 0) ? T(0) : o[k]);
         }
       }
       cp = c[p];
       dp = d[p];
       yp = o[p];
     }
-    if (store && zone != 2)
+    if (!DELAY && store && zone != 2) {
 #pragma unroll
       for (int k = 0; k < p; ++k) put_zeros(ez * p + k, ez, k);
+    }
+  };
+  // the stores (and, CHEBY 5, the prefetch of the next layer's q_s) of the
+  // pending layer.  Vector-memory loads and stores share one completion
+  // counter, and these accesses sit in divergent branches, so the wait for
+  // the next layer's y lines (store_y) is a wait for everything issued
+  // before it: issued at the end of phase Z, the q_s prefetch of CHEBY 5 was
+  // waited for at the start of the next layer.  Issued right after that wait
+  // instead, it has a whole layer to arrive (Q6 64^3 FP64 CG-Chebyshev: the
+  // solve 2 % faster).  The plain matvec and the Chebyshev forms measured
+  // slower this way (the held outputs cost registers: Q6 FP64 matvec 337 ->
+  // 347 us, Chebyshev steps 583 -> 605 us).
+  // (ez: the layer whose outputs are pending, uniform)
+  auto flush = [&](int ez) {
+    if constexpr (DELAY) {
+      if (zone == 0)
+#pragma unroll
+        for (int k = 0; k < p; ++k) put_own(ez, k, po[k]);
+      if (zone != 2)
+#pragma unroll
+        for (int k = 0; k < p; ++k) put_zeros(ez * p + k, ez, k);
+    }
   };
 
   using one = std::integral_constant<int, 1>;
@@ -496,6 +525,8 @@ cont_tp_kernel(const TpArgs<T> a)
   // loads, phases X and Z
   auto layer = [&](int ez, bool store, int ez_next) {
     store_y();
+    // the layer before this one stored its outputs (delayed form)
+    if (DELAY && store && ez > ez_begin) flush(ez - 1);
     __syncthreads();
     if (ez_next >= 0) load_y(ez_next, 1, all());
     phase_x(all());
@@ -520,6 +551,7 @@ cont_tp_kernel(const TpArgs<T> a)
     load_y(ez_begin, 1, all());
   }
   for (int ez = ez_begin; ez < ez_end; ++ez) layer(ez, true, ez + 1 < ez_end ? ez + 1 : -1);
+  if (DELAY && ez_end > ez_begin) flush(ez_end - 1);
   // the grid's last z plane (Dirichlet)
   if (ez_end == a.nz && zone != 2) {
     const int Z = gz - 1;
