@@ -431,11 +431,14 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         return SFK_ERR_NONFINITE;
       }
       const double alpha = rz / pq;
-      // x += alpha p, r -= alpha q, (rs = (float) r), r.r in one pass
-      SOLVER_HIP(vec_cg_update(n, alpha, pv, qv, x, r.p, rs32, S.dres, S.partials, kPartials, s));
+      // r -= alpha q, (rs = (float) r), r.r in one pass; x += alpha p is
+      // folded into the pass that forms the next search direction (which
+      // reads p anyway), or done here when the loop may end
+      SOLVER_HIP(vec_cg_update(n, alpha, nullptr, qv, nullptr, r.p, rs32, S.dres, S.partials, kPartials, s));
       SOLVER_ST(S.results(&rr, 1));
       ++iters;
       res = std::sqrt(rr);
+      if (res <= target || iters == max_iter) SOLVER_HIP(vec_axpby(false, n, alpha, pv, 1., x, s));
       if (res <= target) {
         // the recursively updated residual drifts from b - A x: confirm
         // with the true residual and restart from it if it is not there yet
@@ -456,8 +459,13 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
       SOLVER_ST(precond(r.p, qv, &z64, zrq, &z32));
       const double beta_cg = -alpha * zrq[1] / rz;
       rz = zrq[0];
-      if (z64) SOLVER_HIP(vec_axpby(false, n, 1., z64, beta_cg, pv, s));
-      else SOLVER_HIP(vec_axpby_widen(n, z32, beta_cg, pv, s));
+      // x += alpha p, p = z + beta p
+      if (iters == max_iter) { // x already updated; the loop ends
+        if (z64) SOLVER_HIP(vec_axpby(false, n, 1., z64, beta_cg, pv, s));
+        else SOLVER_HIP(vec_axpby_widen(n, z32, beta_cg, pv, s));
+      } else {
+        SOLVER_HIP(vec_xp_update(z64 == nullptr, n, alpha, z64 ? z64 : z32, beta_cg, pv, x, s));
+      }
     }
   }
   // final FP64 residual

@@ -90,8 +90,8 @@ __global__ void __launch_bounds__(kThreads) dot2_partial_kernel(const double *x,
   }
 }
 
-// CG update x += alpha p, r -= alpha q (FP64), optionally rs = (float) r,
-// with per-block partial sums of r.r
+// CG update x += alpha p (skipped when x is null), r -= alpha q (FP64),
+// optionally rs = (float) r, with per-block partial sums of r.r
 __global__ void __launch_bounds__(kThreads) cg_update_kernel(long long n, double alpha, const double *p,
                                                              const double *q, double *x, double *r, float *rs,
                                                              double *partials)
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kThreads) cg_update_kernel(long long n, double
   __shared__ double red[kThreads];
   double acc = 0.;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
+    if (x) x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     if (rs) rs[i] = float(ri);
@@ -155,6 +155,18 @@ __global__ void axpby_widen_kernel(long long n, const float *zs, double b, doubl
 {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = double(zs[i]) + b * p[i];
+}
+
+// x += a p, then p = z + b p (z FP32 or FP64; x, p FP64): the iterate's
+// update of one CG iteration folded into the next search direction's pass
+template <typename Z>
+__global__ void xp_update_kernel(long long n, double a, const Z *z, double b, double *p, double *x)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double pi = p[i];
+    x[i] += a * pi;
+    p[i] = double(z[i]) + b * pi;
+  }
 }
 
 template <typename T>
@@ -296,6 +308,18 @@ hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, c
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials + g, (int)g,
                      result_dev + 1);
+  return hipGetLastError();
+}
+
+hipError_t vec_xp_update(bool z_fp32, long long n, double a, const void *z, double b, void *p, void *x,
+                         hipStream_t s)
+{
+  if (z_fp32)
+    hipLaunchKernelGGL(xp_update_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, a, (const float *)z, b,
+                       (double *)p, (double *)x);
+  else
+    hipLaunchKernelGGL(xp_update_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, a, (const double *)z,
+                       b, (double *)p, (double *)x);
   return hipGetLastError();
 }
 

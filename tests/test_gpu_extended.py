@@ -144,6 +144,41 @@ def test_cg_chebyshev_solver(cuda, mixed, cheby_degree):
     assert st.lambda_max <= 1.25 * lam.max()
 
 
+@pytest.mark.parametrize("max_iter", [1, 5])
+def test_cg_iterate_at_max_iter(cuda, max_iter):
+    """max_iter reached: the call reports no convergence and x holds the PCG
+    iterate of that many iterations.  Reference: the same flexible PCG in
+    numpy on the oracle operator, preconditioned by D^{-1} (Chebyshev degree
+    1 is (1/theta) D^{-1}; PCG iterates do not change when the
+    preconditioner is scaled)"""
+    import torch
+    degree, nc = 3, (3, 3, 2)
+    ij = (3.0, 3.0, 2.0)
+    op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=ij)
+    b = op.rhs_constant(1.0)
+    x = torch.zeros_like(b)
+    with pytest.raises(sfk.SfkError) as e:
+        sfk.solve_cg_cheby(op, op, b, x, rel_tol=1e-14, max_iter=max_iter, cheby_degree=1, lanczos_steps=12)
+    assert e.value.status == 7
+    d = to_np(op.diagonal())
+    dinv = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+    A = lambda v: ob.cont_laplacian(degree, nc, ij, v)
+    r = to_np(b)
+    xr = np.zeros_like(r)
+    z = dinv * r
+    p, rz = z.copy(), float(r @ z)
+    for _ in range(max_iter):
+        q = A(p)
+        a = rz / float(p @ q)
+        xr += a * p
+        r = r - a * q
+        z = dinv * r
+        beta = -a * float(z @ q) / rz
+        rz = float(z @ r)
+        p = z + beta * p
+    assert rel_err(to_np(x), xr) < 1e-11
+
+
 def test_raw_pointer_wrappers_validate_device_tensors(cuda):
     """wrong dtype or size of a CUDA tensor is rejected before the raw
     pointer reaches the C ABI"""
