@@ -247,15 +247,15 @@ cont_tp_kernel(const TpArgs<T> a)
   const bool ybnd = !(ey0 >= 2 && ey0 + BY < a.ny);
   T u[2 * N - 1];
   bool yact = false;
-  auto load_y = [&](int ez, int k0, auto nkc) {
+  auto load_y_to = [&](T(&uu)[2 * N - 1], bool &act, int ez, int k0, auto nkc) {
     constexpr int NK = decltype(nkc)::value;
     const int it = threadIdx.x;
     const int xh = it % HX, rest = it / HX, eyl = rest % BY, s = rest / BY;
     const int ey = ey0 + eyl;
     // rows beyond the mesh are never read
-    yact = it < NK * BY * HX && ey < a.ny;
+    act = it < NK * BY * HX && ey < a.ny;
     const int X = x0 - p + xh, Z = ez * p + k0 + s;
-    if (yact && X > 0 && X < gx - 1 && Z > 0 && Z < gz - 1) {
+    if (act && X > 0 && X < gx - 1 && Z > 0 && Z < gz - 1) {
       if (!ybnd) {
         // interior tile row range: every row of the line is an interior
         // node row; row j of all lanes is one uniform base + the lane's
@@ -264,18 +264,18 @@ cont_tp_kernel(const TpArgs<T> a)
         const T *lb = src + (long long)(ez * p) * plane;
         const unsigned off = (unsigned)(((k0 + s) * (int)plane + (ey * p - p) * gx + X) * (int)sizeof(T));
 #pragma unroll
-        for (int j = 0; j < 2 * N - 1; ++j) u[j] = at(lb + j * gx, off);
+        for (int j = 0; j < 2 * N - 1; ++j) uu[j] = at(lb + j * gx, off);
       } else {
         const T *col = src + (long long)Z * plane + X;
 #pragma unroll
         for (int j = 0; j < 2 * N - 1; ++j) {
           const int Y = ey * p - p + j;
-          u[j] = (Y > 0 && Y < gy - 1) ? col[(long long)Y * gx] : T(0);
+          uu[j] = (Y > 0 && Y < gy - 1) ? col[(long long)Y * gx] : T(0);
         }
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 2 * N - 1; ++j) u[j] = T(0);
+      for (int j = 0; j < 2 * N - 1; ++j) uu[j] = T(0);
     }
     if constexpr (RSRC) {
       {
@@ -288,17 +288,18 @@ cont_tp_kernel(const TpArgs<T> a)
         const T axz = tb.dk[0][ix] * tb.dm[iz] + tb.dm[ix] * tb.dk[2][iz];
         const T bxz = tb.dm[ix] * tb.dm[iz];
 #pragma unroll
-        for (int j = 0; j < 2 * N - 1; ++j) u[j] *= a.s_rhs / (axz * tb.dm[j % p] + bxz * tb.dk[1][j % p]);
+        for (int j = 0; j < 2 * N - 1; ++j) uu[j] *= a.s_rhs / (axz * tb.dm[j % p] + bxz * tb.dk[1][j % p]);
       }
     }
   };
-  auto store_y = [&]() {
-    if (!yact) return;
+  auto load_y = [&](int ez, int k0, auto nkc) { load_y_to(u, yact, ez, k0, nkc); };
+  auto store_y_from = [&](const T(&uu)[2 * N - 1], bool act) {
+    if (!act) return;
     const int it = threadIdx.x;
     const int xh = it % HX, rest = it / HX, eyl = rest % BY, s = rest / BY;
     T va[N - 1], vb[N - 1];
-    line_one<N, T, 0>(a.tab, u, va); // a = M_y u
-    line_one<N, T, 2>(a.tab, u, vb); // b = c_y K_y u
+    line_one<N, T, 0>(a.tab, uu, va); // a = M_y u
+    line_one<N, T, 2>(a.tab, uu, vb); // b = c_y K_y u
     T *ra = sA + (s * TY + eyl * p) * RPA + xh;
     T *rb = sB + (s * TY + eyl * p) * RPA + xh;
 #pragma unroll
@@ -307,6 +308,7 @@ cont_tp_kernel(const TpArgs<T> a)
       rb[i * RPA] = vb[i];
     }
   };
+  auto store_y = [&]() { store_y_from(u, yact); };
 
   // phase X: rows of slots s < NK
   auto phase_x = [&](auto nkc) {
@@ -535,9 +537,16 @@ cont_tp_kernel(const TpArgs<T> a)
   };
   if (ez_begin > 0) {
     // carried state at plane ez_begin p: c, d of the plane below the cell
-    // below, then that cell's layer without stores
-    load_y(ez_begin - 1, 0, one());
-    store_y();
+    // below, then that cell's layer without stores.  The plane's and the
+    // layer's y lines are loaded together (two sets of registers): one
+    // memory round trip instead of two before the first sweep
+    {
+      T u0[2 * N - 1];
+      bool yact0 = false;
+      load_y_to(u0, yact0, ez_begin - 1, 0, one());
+      load_y(ez_begin - 1, 1, all());
+      store_y_from(u0, yact0);
+    }
     __syncthreads();
     phase_x(one());
     __syncthreads();
@@ -545,7 +554,6 @@ cont_tp_kernel(const TpArgs<T> a)
       cp = sC[Yt * RPC + Xt];
       dp = sD[Yt * RPC + Xt];
     }
-    load_y(ez_begin - 1, 1, all());
     layer(ez_begin - 1, false, ez_begin < ez_end ? ez_begin : -1);
   } else if (ez_begin < ez_end) {
     load_y(ez_begin, 1, all());
