@@ -181,10 +181,12 @@ hipError_t vec_sum_partials(const double *partials, long long count, double *res
 hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, double *result_dev, double *partials,
                     int n_partials, hipStream_t s);
 // solver kernels (FP64 vectors): x += alpha p (x null: skipped), r -= alpha
-// q, optionally rs = (float) r, result_dev[0] = r.r; and z = (double) zs
-// with result_dev[0] = z.r, result_dev[1] = z.q (partials: 2*n_partials)
-hipError_t vec_cg_update(long long n, double alpha, const void *p, const void *q, void *x, void *r, void *rs,
-                         double *result_dev, double *partials, int n_partials, hipStream_t s);
+// q, optionally rs = (float) r, result_dev[0] = r.r (an non-null: alpha =
+// *an / *ad, device scalars); and z = (double) zs with result_dev[0] = z.r,
+// result_dev[1] = z.q (partials: 2*n_partials)
+hipError_t vec_cg_update(long long n, double alpha, const double *an, const double *ad, const void *p,
+                         const void *q, void *x, void *r, void *rs, double *result_dev, double *partials,
+                         int n_partials, hipStream_t s);
 hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, const void *q, double *result_dev,
                           double *partials, int n_partials, hipStream_t s);
 hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, void *y, hipStream_t s);
@@ -192,9 +194,10 @@ hipError_t vec_axpby(bool fp32, long long n, double a, const void *x, double b, 
 hipError_t vec_mul_scaled(bool fp32, long long n, double c, const void *a, const void *b, void *z, hipStream_t s);
 // p = (double) zs + b p (zs FP32, p FP64)
 hipError_t vec_axpby_widen(long long n, const void *zs, double b, void *p, hipStream_t s);
-// x += a p, then p = z + b p (z FP32 when z_fp32, else FP64; x, p FP64)
-hipError_t vec_xp_update(bool z_fp32, long long n, double a, const void *z, double b, void *p, void *x,
-                         hipStream_t s);
+// x += a p, then p = z + b p with a = *an / *ad, b = -*bn / *ad (device
+// scalars; z FP32 when z_fp32, else FP64; x, p FP64)
+hipError_t vec_xp_update(bool z_fp32, long long n, const double *an, const double *ad, const double *bn,
+                         const void *z, void *p, void *x, hipStream_t s);
 hipError_t vec_cheby_update(bool fp32, long long n, const void *au, const void *rhs, const void *sol_old,
                             const void *diag, void *sol_new, void *tm, double c_np, double c_tm,
                             hipStream_t s);

@@ -30,6 +30,7 @@ namespace sfk {
 namespace {
 
 constexpr int kPartials = 2048;
+constexpr int kSlots = 8; // device result slots after the partial sums
 constexpr double kChebyRange = 1000.; // lambda_max / lambda_min of the Chebyshev interval
 
 struct DevBuf {
@@ -37,6 +38,13 @@ struct DevBuf {
   ~DevBuf()
   {
     if (p) (void)hipFree(p);
+  }
+};
+struct PinnedBuf {
+  void *p = nullptr;
+  ~PinnedBuf()
+  {
+    if (p) (void)hipHostFree(p);
   }
 };
 
@@ -106,6 +114,7 @@ struct Solver {
   long long n;
   bool fp32;
   double *partials = nullptr, *dres = nullptr;
+  double *hres = nullptr; // pinned host copy of the result slots
   sfk_solve_stats st{};
 
   sfk_status dot(bool f32, const void *x, const void *y, double &out)
@@ -138,6 +147,20 @@ struct Solver {
         set_error("solver: non-finite inner product");
         return SFK_ERR_NONFINITE;
       }
+    return SFK_OK;
+  }
+
+  // read back `count` result slots (no finiteness check: the caller checks
+  // the slots it uses)
+  sfk_status read_slots(double *out, int count)
+  {
+    hipError_t e = hipMemcpyAsync(hres, dres, count * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error(std::string("solver: ") + hipGetErrorString(e));
+      return SFK_ERR_HIP;
+    }
+    for (int i = 0; i < count; ++i) out[i] = hres[i];
     return SFK_OK;
   }
 
@@ -255,9 +278,13 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
     fill_tp_launch(op, T);
     if (tp_launch_blocks(T) > n_red) n_red = tp_launch_blocks(T);
   }
-  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (n_red + 2)));
+  SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (n_red + kSlots)));
   S.partials = (double *)red.p;
-  S.dres = S.partials + n_red; // after the partial sums: two result slots
+  S.dres = S.partials + n_red; // after the partial sums: the result slots
+  SOLVER_HIP(vec_fill(false, S.dres, kSlots, 0., (hipStream_t)stream));
+  PinnedBuf hres;
+  SOLVER_HIP(hipHostMalloc(&hres.p, sizeof(double) * kSlots));
+  S.hres = (double *)hres.p;
   for (DevBuf *bb : {&d64, &inv64, &isq, &r, &w, &lv0, &lv1, &lw, &lt, &e64})
     SOLVER_HIP(hipMalloc(&bb->p, n8));
   for (DevBuf *bb : {&rs, &zs, &qs, &tms, &inv_s})
@@ -367,17 +394,17 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
 
   // preconditioner z = P(r): the Chebyshev polynomial on the smoother
   // operator, in the smoother's precision.  FP32: rs already holds (float) r.
-  // With zr_zq (the inner products z.r and z.q wanted), an FP32 z stays FP32
-  // (*z32; widened where it is used), otherwise it is widened into e64.
-  // *z64 receives the FP64 buffer that holds z (null when only *z32 does).
-  auto precond = [&](const void *r64, const void *qv, void **z64, double *zr_zq,
+  // With zrq_dev (the inner products z.r and z.q wanted, left in
+  // zrq_dev[0..1] on the device), an FP32 z stays FP32 (*z32; widened where
+  // it is used), otherwise it is widened into e64.  *z64 receives the FP64
+  // buffer that holds z (null when only *z32 does).
+  auto precond = [&](const void *r64, const void *qv, void **z64, double *zrq_dev,
                      const void **z32 = nullptr) -> sfk_status {
     if (f32) {
       void *z = zs.p, *spare = qs.p;
       SOLVER_ST(cheby(rs.p, &z, &spare));
-      if (zr_zq) {
-        SOLVER_HIP(vec_widen_dot2(n, z, nullptr, r64, qv, S.dres, S.partials, kPartials, s));
-        SOLVER_ST(S.results(zr_zq, 2));
+      if (zrq_dev) {
+        SOLVER_HIP(vec_widen_dot2(n, z, nullptr, r64, qv, zrq_dev, S.partials, kPartials, s));
         *z64 = nullptr;
         *z32 = z;
         return SFK_OK;
@@ -388,7 +415,7 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
       void *z = e64.p, *spare = zs.p;
       SOLVER_ST(cheby(r64, &z, &spare));
       *z64 = z;
-      if (zr_zq) SOLVER_ST(S.dot2(z, r64, qv, zr_zq[0], zr_zq[1]));
+      if (zrq_dev) SOLVER_HIP(vec_dot2(z, r64, qv, n, zrq_dev, S.partials, kPartials, s));
     }
     return SFK_OK;
   };
@@ -396,21 +423,35 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   // PCG in FP64 (x, r, p, q, z and all inner products FP64).  The FP32
   // polynomial preconditioner is linear only up to FP32 rounding, so the
   // search direction uses the flexible (Polak-Ribiere) coefficient
-  // beta = z_new . (r_new - r_old) / (z_old . r_old) = -alpha (z_new . q) / rz.
+  // beta = z_new . (r_new - r_old) / (z_old . r_old) = -(z_new . q) / (p . q).
   // (Orthogonalising against the last two directions as well, flexible CG
   // with a truncation of two, did not reduce the iteration count on the
   // Q6 64^3 problem and costs two more vector passes per iteration.)
+  // The step coefficients stay on the device: the kernels that use alpha =
+  // (z.r) / (p.q) and beta read the inner products from the result slots,
+  // and the host reads p.q, r.r and z.r back once per iteration (for the
+  // convergence test), instead of waiting for every inner product.
+  // Slots: 0 p.q, 1 r.r, 2 + 2 (k & 1) and 3 + 2 (k & 1) z.r and z.q of
+  // the search direction of iteration k (the next iteration's pair is
+  // written while this one's z.r is still needed for x += alpha p).
   const double target = rel_tol * bnorm;
+  double *sl = S.dres;
+  auto zr_slot = [&](int k) { return sl + 2 + 2 * (k & 1); };
   if (res > target) {
     void *z64 = nullptr, *pv = w.p, *qv = lt.p;
     void *rs32 = f32 ? rs.p : nullptr; // the CG update also writes (float) r
-    if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rs.p, n, s));
-    SOLVER_ST(precond(r.p, nullptr, &z64, nullptr));
-    SOLVER_HIP(hipMemcpyAsync(pv, z64, n8, hipMemcpyDeviceToDevice, s));
-    double rz;
-    SOLVER_ST(S.dot(false, r.p, z64, rz));
+    // a (re)start from the residual in r.p: p = z = P(r), z.r into the slot
+    // of iteration k
+    auto restart = [&](int k) -> sfk_status {
+      if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rs.p, n, s));
+      SOLVER_ST(precond(r.p, nullptr, &z64, nullptr));
+      SOLVER_HIP(hipMemcpyAsync(pv, z64, n8, hipMemcpyDeviceToDevice, s));
+      SOLVER_HIP(vec_dot(false, r.p, z64, n, zr_slot(k), S.partials, kPartials, s));
+      return SFK_OK;
+    };
+    SOLVER_ST(restart(0));
     while (iters < max_iter) {
-      double pq;
+      // q = A p and p.q -> slot 0
       if (fused_pq) {
         SOLVER_ST(ensure_device_tables(op));
         TpLaunch T;
@@ -419,23 +460,31 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         T.dst = qv;
         T.dot_part = S.partials;
         SOLVER_HIP(launch_cont_tp(T, s));
-        SOLVER_HIP(vec_sum_partials(S.partials, tp_launch_blocks(T), S.dres, s));
-        SOLVER_ST(S.results(&pq, 1));
+        SOLVER_HIP(vec_sum_partials(S.partials, tp_launch_blocks(T), sl, s));
       } else {
         SOLVER_ST(sfk_apply(op, pv, qv, stream));
-        SOLVER_ST(S.dot(false, pv, qv, pq));
+        SOLVER_HIP(vec_dot(false, pv, qv, n, sl, S.partials, kPartials, s));
       }
       ++S.st.matvecs_outer;
+      // r -= alpha q with alpha = (z.r) / (p.q) from the slots, (rs =
+      // (float) r), r.r -> slot 1 in one pass; x += alpha p is folded into
+      // the pass that forms the next search direction (which reads p
+      // anyway), or done below when the loop may end
+      double *zr = zr_slot(iters);
+      SOLVER_HIP(vec_cg_update(n, 0., zr, sl, nullptr, qv, nullptr, r.p, rs32, sl + 1, S.partials, kPartials, s));
+      double h[2 + 4];
+      SOLVER_ST(S.read_slots(h, 6));
+      const double pq = h[0], rz = h[2 + 2 * (iters & 1)];
+      rr = h[1];
+      if (!std::isfinite(pq) || !std::isfinite(rr) || !std::isfinite(rz)) {
+        set_error("solver: non-finite inner product");
+        return SFK_ERR_NONFINITE;
+      }
       if (!(pq > 0.)) {
         set_error("sfk_solve_cg_cheby: operator not positive definite on the search direction");
         return SFK_ERR_NONFINITE;
       }
       const double alpha = rz / pq;
-      // r -= alpha q, (rs = (float) r), r.r in one pass; x += alpha p is
-      // folded into the pass that forms the next search direction (which
-      // reads p anyway), or done here when the loop may end
-      SOLVER_HIP(vec_cg_update(n, alpha, nullptr, qv, nullptr, r.p, rs32, S.dres, S.partials, kPartials, s));
-      SOLVER_ST(S.results(&rr, 1));
       ++iters;
       res = std::sqrt(rr);
       if (res <= target || iters == max_iter) SOLVER_HIP(vec_axpby(false, n, alpha, pv, 1., x, s));
@@ -448,23 +497,26 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
         SOLVER_ST(S.dot(false, r.p, r.p, rr));
         res = std::sqrt(rr);
         if (res <= target) break;
-        if (f32) SOLVER_HIP(vec_copy_convert(true, r.p, rs.p, n, s));
-        SOLVER_ST(precond(r.p, nullptr, &z64, nullptr));
-        SOLVER_HIP(hipMemcpyAsync(pv, z64, n8, hipMemcpyDeviceToDevice, s));
-        SOLVER_ST(S.dot(false, r.p, z64, rz));
+        SOLVER_ST(restart(iters));
         continue;
       }
-      double zrq[2];
+      // z = P(r); z.r and z.q -> the slots of the next iteration
       const void *z32 = nullptr;
+      double *zrq = zr_slot(iters);
       SOLVER_ST(precond(r.p, qv, &z64, zrq, &z32));
-      const double beta_cg = -alpha * zrq[1] / rz;
-      rz = zrq[0];
-      // x += alpha p, p = z + beta p
-      if (iters == max_iter) { // x already updated; the loop ends
+      const void *zv = z64 ? (const void *)z64 : z32;
+      if (iters == max_iter) {
+        // the loop ends: x is up to date; p = z + beta p for completeness
+        double hz[2];
+        SOLVER_HIP(hipMemcpyAsync(hz, zrq, sizeof hz, hipMemcpyDeviceToHost, s));
+        SOLVER_HIP(hipStreamSynchronize(s));
+        const double beta_cg = -hz[1] / pq;
         if (z64) SOLVER_HIP(vec_axpby(false, n, 1., z64, beta_cg, pv, s));
         else SOLVER_HIP(vec_axpby_widen(n, z32, beta_cg, pv, s));
       } else {
-        SOLVER_HIP(vec_xp_update(z64 == nullptr, n, alpha, z64 ? z64 : z32, beta_cg, pv, x, s));
+        // x += alpha p, p = z + beta p: alpha = (z.r) / (p.q) of this
+        // iteration's direction, beta = -(z_new . q) / (p.q)
+        SOLVER_HIP(vec_xp_update(z64 == nullptr, n, zr, sl, zrq + 1, zv, pv, x, s));
       }
     }
   }

@@ -91,12 +91,14 @@ __global__ void __launch_bounds__(kThreads) dot2_partial_kernel(const double *x,
 }
 
 // CG update x += alpha p (skipped when x is null), r -= alpha q (FP64),
-// optionally rs = (float) r, with per-block partial sums of r.r
-__global__ void __launch_bounds__(kThreads) cg_update_kernel(long long n, double alpha, const double *p,
-                                                             const double *q, double *x, double *r, float *rs,
-                                                             double *partials)
+// optionally rs = (float) r, with per-block partial sums of r.r; with an
+// non-null, alpha = *an / *ad (device scalars: no host round trip)
+__global__ void __launch_bounds__(kThreads) cg_update_kernel(long long n, double alpha, const double *an,
+                                                             const double *ad, const double *p, const double *q,
+                                                             double *x, double *r, float *rs, double *partials)
 {
   __shared__ double red[kThreads];
+  if (an) alpha = *an / *ad;
   double acc = 0.;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     if (x) x[i] += alpha * p[i];
@@ -158,10 +160,13 @@ __global__ void axpby_widen_kernel(long long n, const float *zs, double b, doubl
 }
 
 // x += a p, then p = z + b p (z FP32 or FP64; x, p FP64): the iterate's
-// update of one CG iteration folded into the next search direction's pass
+// update of one CG iteration folded into the next search direction's pass.
+// a = *an / *ad, b = -*bn / *ad (device scalars)
 template <typename Z>
-__global__ void xp_update_kernel(long long n, double a, const Z *z, double b, double *p, double *x)
+__global__ void xp_update_kernel(long long n, const double *an, const double *ad, const double *bn, const Z *z,
+                                 double *p, double *x)
 {
+  const double a = *an / *ad, b = -*bn / *ad;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double pi = p[i];
     x[i] += a * pi;
@@ -287,13 +292,14 @@ hipError_t vec_dot2(const void *x, const void *y, const void *w, long long n, do
   return hipGetLastError();
 }
 
-hipError_t vec_cg_update(long long n, double alpha, const void *p, const void *q, void *x, void *r, void *rs,
-                         double *result_dev, double *partials, int n_partials, hipStream_t s)
+hipError_t vec_cg_update(long long n, double alpha, const double *an, const double *ad, const void *p,
+                         const void *q, void *x, void *r, void *rs, double *result_dev, double *partials,
+                         int n_partials, hipStream_t s)
 {
   unsigned g = grid_for(n, 8);
   if ((int)g > n_partials) g = n_partials;
-  hipLaunchKernelGGL(cg_update_kernel, dim3(g), dim3(kThreads), 0, s, n, alpha, (const double *)p, (const double *)q,
-                     (double *)x, (double *)r, (float *)rs, partials);
+  hipLaunchKernelGGL(cg_update_kernel, dim3(g), dim3(kThreads), 0, s, n, alpha, an, ad, (const double *)p,
+                     (const double *)q, (double *)x, (double *)r, (float *)rs, partials);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, (const double *)partials, (int)g, result_dev);
   return hipGetLastError();
 }
@@ -311,15 +317,15 @@ hipError_t vec_widen_dot2(long long n, const void *zs, void *z, const void *r, c
   return hipGetLastError();
 }
 
-hipError_t vec_xp_update(bool z_fp32, long long n, double a, const void *z, double b, void *p, void *x,
-                         hipStream_t s)
+hipError_t vec_xp_update(bool z_fp32, long long n, const double *an, const double *ad, const double *bn,
+                         const void *z, void *p, void *x, hipStream_t s)
 {
   if (z_fp32)
-    hipLaunchKernelGGL(xp_update_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, a, (const float *)z, b,
-                       (double *)p, (double *)x);
+    hipLaunchKernelGGL(xp_update_kernel<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, an, ad, bn,
+                       (const float *)z, (double *)p, (double *)x);
   else
-    hipLaunchKernelGGL(xp_update_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, a, (const double *)z,
-                       b, (double *)p, (double *)x);
+    hipLaunchKernelGGL(xp_update_kernel<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, s, n, an, ad, bn,
+                       (const double *)z, (double *)p, (double *)x);
   return hipGetLastError();
 }
 
