@@ -726,18 +726,23 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
 
 // tile shapes (BX, BY) per points per direction N; shape 0 is the default
 #define SFK_TP_SHAPES(X)                                                                       \
-  X(2, 0, 32, 16) X(2, 1, 64, 8) X(2, 2, 16, 32) X(2, 3, 32, 8)                                 \
-  X(3, 0, 16, 8) X(3, 1, 32, 4) X(3, 2, 8, 16) X(3, 3, 16, 4)                                   \
-  X(4, 0, 8, 8) X(4, 1, 16, 4) X(4, 2, 4, 16) X(4, 3, 8, 4)                                     \
-  X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 8, 2) X(5, 3, 16, 1)                                     \
-  X(6, 0, 4, 4) X(6, 1, 8, 2) X(6, 2, 8, 4) X(6, 3, 4, 2)                                       \
-  X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2) X(7, 3, 8, 1)                                       \
-  X(8, 0, 4, 2) X(8, 1, 2, 4) X(8, 2, 2, 2) X(8, 3, 4, 1)                                       \
-  X(9, 0, 4, 2) X(9, 1, 2, 2) X(9, 2, 4, 1) X(9, 3, 2, 1)
+  X(2, 0, 32, 16) X(2, 1, 64, 8) X(2, 2, 16, 32) X(2, 3, 32, 8) X(2, 4, 128, 4)                 \
+  X(3, 0, 16, 8) X(3, 1, 32, 4) X(3, 2, 8, 16) X(3, 3, 16, 4) X(3, 4, 64, 2)                    \
+  X(4, 0, 8, 8) X(4, 1, 16, 4) X(4, 2, 4, 16) X(4, 3, 8, 4) X(4, 4, 32, 2)                      \
+  X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 8, 2) X(5, 3, 16, 1) X(5, 4, 32, 1)                      \
+  X(6, 0, 4, 4) X(6, 1, 8, 2) X(6, 2, 8, 4) X(6, 3, 4, 2) X(6, 4, 16, 1)                        \
+  X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2) X(7, 3, 8, 1) X(7, 4, 16, 1)                        \
+  X(8, 0, 4, 2) X(8, 1, 2, 4) X(8, 2, 2, 2) X(8, 3, 4, 1) X(8, 4, 8, 1)                         \
+  X(9, 0, 4, 2) X(9, 1, 2, 2) X(9, 2, 4, 1) X(9, 3, 2, 1) X(9, 4, 8, 1)
 
-constexpr int kTpShapes = 4;
-
-int tp_shape_count(int) { return kTpShapes; }
+int tp_shape_count(int n)
+{
+  int c = 0;
+#define SFK_X(NN, SH, BXX, BYY) c += n == NN ? 1 : 0;
+  SFK_TP_SHAPES(SFK_X)
+#undef SFK_X
+  return c;
+}
 
 void tp_shape(int n, int shape, int *bx, int *by)
 {
@@ -764,7 +769,7 @@ size_t tp_lds_bytes(int n, int shape, bool fp32)
 template <typename T>
 static hipError_t launch_tp_t(const TpLaunch &L, hipStream_t s)
 {
-  const int shape = L.shape >= 0 && L.shape < kTpShapes ? L.shape : 0;
+  const int shape = L.shape >= 0 && L.shape < tp_shape_count(L.n) ? L.shape : 0;
 #define SFK_X(NN, SH, BXX, BYY) \
   if (L.n == NN && shape == SH) return launch_tp_shape<NN, T, BXX, BYY>(L, s);
   SFK_TP_SHAPES(SFK_X)
