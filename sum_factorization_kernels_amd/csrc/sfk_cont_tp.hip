@@ -731,7 +731,7 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   X(4, 0, 8, 8) X(4, 1, 16, 4) X(4, 2, 4, 16) X(4, 3, 8, 4) X(4, 4, 32, 2)                      \
   X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 8, 2) X(5, 3, 16, 1) X(5, 4, 32, 1)                      \
   X(6, 0, 4, 4) X(6, 1, 8, 2) X(6, 2, 8, 4) X(6, 3, 4, 2) X(6, 4, 16, 1)                        \
-  X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2) X(7, 3, 8, 1) X(7, 4, 16, 1) X(7, 5, 24, 1)        \
+  X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2) X(7, 3, 8, 1) X(7, 4, 16, 1) X(7, 5, 24, 1) X(7, 6, 22, 1) \
   X(8, 0, 4, 2) X(8, 1, 2, 4) X(8, 2, 2, 2) X(8, 3, 4, 1) X(8, 4, 8, 1)                         \
   X(9, 0, 4, 2) X(9, 1, 2, 2) X(9, 2, 4, 1) X(9, 3, 2, 1) X(9, 4, 8, 1)
 

@@ -270,12 +270,15 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
   DevBuf red, d64, inv64, isq, r, w, lv0, lv1, lw, lt, e64;
   DevBuf rs, zs, qs, tms, inv_s;
   // the outer matvec on the tensor-product kernel also sums p . q (its
-  // workgroups' partial sums, one per workgroup)
+  // workgroups' partial sums, one per workgroup).  It runs on the schedule
+  // tuned for the operator's fused Chebyshev launches: like them it holds
+  // more per-node state than the plain matvec, which the widest tiles
+  // cannot hold without spills (Q6 FP64 22 x 1 cells: 128 VGPRs + spills)
   const bool fused_pq = uses_tp(op) && op->d.n_rhs == 1;
   long long n_red = 2 * kPartials;
   if (fused_pq) {
     TpLaunch T;
-    fill_tp_launch(op, T);
+    fill_tp_launch(op, T, true);
     if (tp_launch_blocks(T) > n_red) n_red = tp_launch_blocks(T);
   }
   SOLVER_HIP(hipMalloc(&red.p, sizeof(double) * (n_red + kSlots)));
@@ -455,7 +458,7 @@ extern "C" sfk_status sfk_solve_cg_cheby(sfk_op *op, sfk_op *smoother_op, const 
       if (fused_pq) {
         SOLVER_ST(ensure_device_tables(op));
         TpLaunch T;
-        fill_tp_launch(op, T);
+        fill_tp_launch(op, T, true);
         T.src = pv;
         T.dst = qv;
         T.dot_part = S.partials;
