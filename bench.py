@@ -202,7 +202,7 @@ def cpu_dg_advect(degree, n_cells, threads, target_seconds=5.0):
 
 
 # ------------------------------------------------------------ workloads
-def cont_op(degree, n, prec, tune=True, repeats=3, **kw):
+def cont_op(degree, n, prec, tune=True, repeats=10, **kw):
     import sum_factorization_kernels_amd as sfk
     nc = (n, n, n)
     op = sfk.Operator(sfk.CONT_LAPLACE, degree, nc, inv_jac=(float(n),) * 3, precision=prec, **kw)
