@@ -729,7 +729,7 @@ static hipError_t launch_tp_shape(const TpLaunch &L, hipStream_t s)
   X(2, 0, 32, 16) X(2, 1, 64, 8) X(2, 2, 16, 32) X(2, 3, 32, 8) X(2, 4, 128, 4)                 \
   X(3, 0, 16, 8) X(3, 1, 32, 4) X(3, 2, 8, 16) X(3, 3, 16, 4) X(3, 4, 64, 2)                    \
   X(4, 0, 8, 8) X(4, 1, 16, 4) X(4, 2, 4, 16) X(4, 3, 8, 4) X(4, 4, 32, 2)                      \
-  X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 8, 2) X(5, 3, 16, 1) X(5, 4, 32, 1)                      \
+  X(5, 0, 8, 4) X(5, 1, 16, 2) X(5, 2, 8, 2) X(5, 3, 16, 1) X(5, 4, 32, 1) X(5, 5, 43, 1)                \
   X(6, 0, 4, 4) X(6, 1, 8, 2) X(6, 2, 8, 4) X(6, 3, 4, 2) X(6, 4, 16, 1)                        \
   X(7, 0, 4, 4) X(7, 1, 4, 2) X(7, 2, 8, 2) X(7, 3, 8, 1) X(7, 4, 16, 1) X(7, 5, 24, 1) X(7, 6, 22, 1) \
   X(8, 0, 4, 2) X(8, 1, 2, 4) X(8, 2, 2, 2) X(8, 3, 4, 1) X(8, 4, 8, 1)                         \
