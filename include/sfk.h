@@ -190,7 +190,8 @@ typedef struct sfk_schedule {
                           assembled 1D mass/stiffness sweeps on the node
                           grid, no scatter; the automatic choice then) */
   int tile_shape;      /* cont_scatter 2: tile of cells in x/y per workgroup,
-                          an index < sfk_op_tile_shapes (0 = default) */
+                          an index < the count sfk_op_tile_shape returns
+                          (0 = default) */
   int z_cells;         /* cont_scatter 2: cells in z per workgroup (0 =
                           automatic) */
 } sfk_schedule;
@@ -203,7 +204,9 @@ sfk_status sfk_op_get_schedule(const sfk_op *op, sfk_schedule *s);
 sfk_status sfk_op_get_smoother_schedule(const sfk_op *op, sfk_schedule *s);
 /* Sets the schedule (of the matvec and of the Chebyshev launches);
  * SFK_ERR_INVALID_ARG if cells_per_block exceeds
- * sfk_op_max_cells_per_block. */
+ * sfk_op_max_cells_per_block; SFK_ERR_UNSUPPORTED if the tile shape needs
+ * more LDS than a workgroup has in the operator's precision (the widest Q6
+ * tile fits FP32 only; sfk_tune skips such shapes). */
 sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s);
 /* Largest cells_per_block of the operator (workgroup of at most 256 threads,
  * LDS within the device's per-workgroup limit). */
