@@ -18,7 +18,9 @@ coefficient gradient, and the precomputed-matrix DG Laplacian p=3.
 
 Timing: HIP events (torch.cuda.Event) on the stream every kernel is launched
 on (torch's current stream, passed through the C ABI), inputs resident in
-HBM.  roofline.achieved = algorithmic bytes per launch / mean launch time;
+HBM; before the W warm-up steps, 50 ms of untimed calls bring the GPU to
+its clocks under sustained load (settle()), so that K steps measure the
+steady state also when K and W are small.  roofline.achieved = algorithmic bytes per launch / mean launch time;
 the algorithmic bytes count every vector and stored array the operation must
 touch once, at its stored precision (the rule of the reference's drivers,
 legacy/src/test_dg_integrate.cc:127,183,235).
@@ -46,8 +48,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def settle(fn, seconds=0.05):
+    """untimed calls of fn for about `seconds` of sustained GPU load: the
+    MI355X raises its clocks only under load, and the first ~60 launches
+    after any pause run up to 20 % slower (Q6 64^3 FP64 matvec: 339, 305,
+    294, 285 us over consecutive batches of 20), so the timed steps follow
+    this, the warm-up steps and the autotuner's trials"""
+    import torch
+    t_end = time.perf_counter() + seconds
+    while True:
+        for _ in range(8):
+            fn()
+        torch.cuda.synchronize()
+        if time.perf_counter() >= t_end:
+            return
+
+
 def timed(fn, steps, warmup):
     import torch
+    settle(fn)
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
