@@ -512,7 +512,7 @@ sfk_status sfk_cheby_step(sfk_op *op, const void *rhs, const void *sol_old, cons
       SFK_REQUIRE(op->cell_metric && op->face_metric, SFK_ERR_NOT_READY, "metric not set");
       SipgLaunch L;
       SFK_ENSURE_TABLES(op);
-      fill_sipg_launch(op, L);
+      fill_sipg_launch(op, L, true);
       L.cheby = true;
       L.src = sol_old;
       L.dst = sol_new;

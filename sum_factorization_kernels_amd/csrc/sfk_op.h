@@ -29,6 +29,9 @@ struct sfk_op {
   // solver's smoother): tile shape, cells in z and block order chosen by
   // sfk_tune for those launches (cheby_shape -1: the matvec's)
   int cheby_shape = -1, cheby_zc = 0, cheby_order = 1;
+  // the per-q-metric SIPG kernel's fused Chebyshev step: cells per block
+  // chosen by sfk_tune for it, with cheby_order (0: the matvec's schedule)
+  int cheby_cb = 0;
   void *tptab = nullptr; // TpTab of the tensor-product kernel (constant coefficient)
   // precomputed DG Laplacian: neighbour coupling constants not both zero
   // (set with the device tables)
@@ -37,7 +40,7 @@ struct sfk_op {
 
 namespace sfk {
 
-inline void fill_sipg_launch(const sfk_op *op, SipgLaunch &L)
+inline void fill_sipg_launch(const sfk_op *op, SipgLaunch &L, bool cheby = false)
 {
   L.tables = &op->tab;
   L.fp32 = op->d.precision == SFK_FP32;
@@ -48,6 +51,10 @@ inline void fill_sipg_launch(const sfk_op *op, SipgLaunch &L)
   L.dtab = op->dtab;
   L.cb = op->sched_cb;
   L.order = op->sched_order;
+  if (cheby && op->cheby_cb > 0) {
+    L.cb = op->cheby_cb;
+    L.order = op->cheby_order;
+  }
 }
 
 inline void fill_advect_launch(const sfk_op *op, AdvectLaunch &L)

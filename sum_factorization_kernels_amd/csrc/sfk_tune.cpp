@@ -67,6 +67,11 @@ int lds_limit()
   return lds_max;
 }
 
+bool sipg_per_q(const sfk_op *op)
+{
+  return op->d.kind == SFK_DG_SIPG && op->d.geometry == SFK_GEOM_PER_Q;
+}
+
 bool tp_ok(const sfk_op *op)
 {
   return op->d.kind == SFK_CONT_LAPLACE && !(op->d.flags & SFK_FLAG_VARIABLE_COEF) && op->d.degree <= kMaxDegreeTp;
@@ -171,6 +176,10 @@ sfk_status sfk_op_get_smoother_schedule(const sfk_op *op, sfk_schedule *s)
     s->tile_shape = op->cheby_shape;
     s->z_cells = op->cheby_zc;
   }
+  if (op->cheby_cb > 0) {
+    s->cells_per_block = op->cheby_cb;
+    s->block_order = op->cheby_order;
+  }
   return SFK_OK;
 }
 
@@ -219,6 +228,7 @@ sfk_status sfk_op_set_schedule(sfk_op *op, const sfk_schedule *s)
   op->tp_shape = s->tile_shape;
   op->tp_zc = s->z_cells;
   op->cheby_shape = -1; // the Chebyshev launches follow the new schedule
+  op->cheby_cb = 0;
   return SFK_OK;
 }
 
@@ -233,11 +243,17 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
     if (cache_lookup(cache, key, e)) {
       // validated like a hand-set schedule; a stale line falls through to tuning
       if (sfk_op_set_schedule(op, &e.s) == SFK_OK) {
-        const bool cheby_ok = e.cheby[0] < 0 || (tp_ok(op) && e.cheby[0] < tp_shape_count(op->d.degree + 1) &&
-                                                 e.cheby[1] >= 0 && (e.cheby[2] == 0 || e.cheby[2] == 1));
+        // per-q SIPG: the Chebyshev step's cells per block in the z-cells
+        // field (shape -1), orders 0..2
+        const bool dg_cheby = sipg_per_q(op) && e.cheby[0] < 0 && e.cheby[1] > 0;
+        const bool cheby_ok =
+            dg_cheby ? e.cheby[1] <= max_cb(op) && e.cheby[2] >= 0 && e.cheby[2] <= 2
+                     : e.cheby[0] < 0 || (tp_ok(op) && e.cheby[0] < tp_shape_count(op->d.degree + 1) &&
+                                          e.cheby[1] >= 0 && (e.cheby[2] == 0 || e.cheby[2] == 1));
         if (cheby_ok) {
-          op->cheby_shape = e.cheby[0];
-          op->cheby_zc = e.cheby[1];
+          op->cheby_shape = dg_cheby ? -1 : e.cheby[0];
+          op->cheby_zc = dg_cheby ? 0 : e.cheby[1];
+          op->cheby_cb = dg_cheby ? e.cheby[1] : 0;
           op->cheby_order = e.cheby[2];
           if (best) *best = e.s;
           if (best_seconds) *best_seconds = e.seconds;
@@ -254,6 +270,7 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
   const int saved_cb = op->sched_cb, saved_order = op->sched_order, saved_scatter = op->cont_scatter;
   const int saved_shape = op->tp_shape, saved_zc = op->tp_zc;
   const int saved_cshape = op->cheby_shape, saved_czc = op->cheby_zc, saved_corder = op->cheby_order;
+  const int saved_ccb = op->cheby_cb;
   sfk_schedule win{saved_cb > 0 ? saved_cb : default_cb(op), saved_order, saved_scatter, saved_shape, saved_zc};
   double win_t = -1.;
   if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
@@ -376,6 +393,47 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
     }
     if (pv) (void)hipFree(pv);
   }
+  // the per-q SIPG kernel's fused Chebyshev step (six streams and the
+  // metric: its best cells per block and order differ from the matvec's)
+  int dwin[2] = {0, 1};
+  if (status == SFK_OK && sipg_per_q(op)) {
+    void *tv = nullptr;
+    if (hipMalloc(&tv, bytes) != hipSuccess) {
+      set_error("sfk_tune: scratch allocation failed");
+      status = SFK_ERR_ALLOC;
+    }
+    if (status == SFK_OK) status = sfk_vec_fill(op->d.precision, tv, op->nvec, 0.0, stream);
+    double dwin_t = -1.;
+    const int m = max_cb(op);
+    for (int order = 0; order <= 2 && status == SFK_OK; ++order)
+      for (int cb = 1; cb <= m && status == SFK_OK; ++cb) {
+        op->cheby_cb = cb;
+        op->cheby_order = order;
+        // rhs = diag = input = x (ones), tmp tv (read: c_tm != 0)
+        status = sfk_cheby_step(op, x, x, x, y, tv, 0.5, 0.25, stream); // warm-up
+        if (status != SFK_OK) break;
+        float ms = 0.f;
+        bool ok = hipEventRecord(e0, s) == hipSuccess;
+        for (int r = 0; r < repeats && ok && status == SFK_OK; ++r)
+          status = sfk_cheby_step(op, x, x, x, y, tv, 0.5, 0.25, stream);
+        ok = ok && status == SFK_OK && hipEventRecord(e1, s) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (!ok) {
+          if (status == SFK_OK) {
+            set_error("sfk_tune: Chebyshev timing failed");
+            status = SFK_ERR_HIP;
+          }
+          break;
+        }
+        const double t = 1e-3 * ms / repeats;
+        if (dwin_t < 0. || t < dwin_t) {
+          dwin_t = t;
+          dwin[0] = cb;
+          dwin[1] = order;
+        }
+      }
+    if (tv) (void)hipFree(tv);
+  }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (x) (void)hipFree(x);
@@ -389,12 +447,22 @@ sfk_status sfk_tune(sfk_op *op, int repeats, sfk_schedule *best, double *best_se
     op->cheby_shape = saved_cshape;
     op->cheby_zc = saved_czc;
     op->cheby_order = saved_corder;
+    op->cheby_cb = saved_ccb;
     return status != SFK_OK ? status : SFK_ERR_HIP;
   }
   op->cheby_shape = cwin[0];
   op->cheby_zc = cwin[1];
   op->cheby_order = cwin[2];
-  if (!key.empty()) cache_store(cache, key, CacheEntry{win, {cwin[0], cwin[1], cwin[2]}, win_t});
+  op->cheby_cb = 0;
+  if (dwin[0] > 0) {
+    op->cheby_cb = dwin[0];
+    op->cheby_order = dwin[1];
+  }
+  if (!key.empty()) {
+    const CacheEntry ce = dwin[0] > 0 ? CacheEntry{win, {-1, dwin[0], dwin[1]}, win_t}
+                                      : CacheEntry{win, {cwin[0], cwin[1], cwin[2]}, win_t};
+    cache_store(cache, key, ce);
+  }
   op->sched_cb = win.cells_per_block;
   op->sched_order = win.block_order;
   op->cont_scatter = win.cont_scatter;

@@ -365,3 +365,46 @@ def test_cheby_precondition_matches_sweep(cuda, prec, degree, p):
     op.cheby_sweep(rhs, x, spare, dinv, tm, degree, lmin, lmax)
     tol = 5e-5 if prec == "fp32" else 1e-10
     assert rel_err(to_np(z), to_np(x)) < tol
+
+
+def test_tune_per_q_sipg_chebyshev_schedule(cuda, tmp_path, monkeypatch):
+    """per-q SIPG: sfk_tune also picks the fused Chebyshev step's cells per
+    block and order (smoother schedule), the step on it matches the oracle,
+    set_schedule drops it, and the tune cache restores it"""
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("SFK_TUNE_CACHE", str(path))
+    degree, nc = 3, (4, 3, 5)
+    cm, fm = sine_metric(degree, nc, 0.1)
+    N = (degree + 1) ** 3 * int(np.prod(nc))
+    rng = np.random.default_rng(4510)
+    src, rhs = rng.standard_normal(N), rng.standard_normal(N)
+    diag, tm = rng.uniform(0.5, 1.5, N), rng.standard_normal(N)
+    ref = ob.dg_sipg_general(degree, nc, cm, fm, src)
+    r_tm = 0.45 * tm + 0.6 * diag * (ref - rhs)
+
+    def make():
+        op = sfk.Operator(sfk.DG_SIPG, degree, nc, geometry=sfk.GEOM_PER_Q)
+        op.set_cell_metric(cm)
+        op.set_face_metric(fm)
+        return op
+
+    def check(op):
+        d_new, d_tm = to_dev(np.zeros(N)), to_dev(tm)
+        op.cheby_step(to_dev(rhs), to_dev(src), to_dev(diag), d_new, d_tm, 0.6, 0.45)
+        assert rel_err(to_np(d_tm), r_tm) < TOL_FP64
+        assert rel_err(to_np(d_new), src - r_tm) < TOL_FP64
+
+    op = make()
+    sched, _ = op.tune(repeats=2)
+    sm = op.smoother_schedule
+    assert 1 <= sm[0] <= op.max_cells_per_block and 0 <= sm[1] <= 2
+    check(op)
+    f = path.read_text().split()
+    assert int(f[6]) == -1 and (int(f[7]), int(f[8])) == (sm[0], sm[1])
+    op2 = make()
+    op2.tune(repeats=2)  # from the cache line
+    assert op2.smoother_schedule == sm
+    check(op2)
+    op2.set_schedule(*sched)
+    assert op2.smoother_schedule == op2.schedule
+    check(op2)
