@@ -198,9 +198,11 @@ typedef struct sfk_schedule {
 
 /* Current schedule, with defaults resolved (cells_per_block > 0). */
 sfk_status sfk_op_get_schedule(const sfk_op *op, sfk_schedule *s);
-/* Schedule of the tensor-product kernel's fused Chebyshev launches
- * (sfk_cheby_step and the smoother of sfk_solve_cg_cheby on this operator):
- * the one sfk_tune picked for them, else the matvec schedule. */
+/* Schedule of the fused Chebyshev launches (sfk_cheby_step and the smoother
+ * of sfk_solve_cg_cheby on this operator) of the tensor-product kernel (tile
+ * shape, z_cells, order) and of the per-q-metric SIPG kernel (cells per
+ * block, order): the one sfk_tune picked for them, else the matvec
+ * schedule. */
 sfk_status sfk_op_get_smoother_schedule(const sfk_op *op, sfk_schedule *s);
 /* Sets the schedule (of the matvec and of the Chebyshev launches);
  * SFK_ERR_INVALID_ARG if cells_per_block exceeds
