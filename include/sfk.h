@@ -225,7 +225,9 @@ sfk_status sfk_op_tile_shape(const sfk_op *op, int shape, int *count, int *bx, i
  * x z_cells (powers of two up to the cells in z) of the tensor-product
  * kernel; for that kernel the fused Chebyshev launch (three-term form, four
  * vector streams) is timed over the same tile shapes, z_cells and orders and
- * its fastest kept separately (sfk_op_get_smoother_schedule).  Requires the
+ * its fastest kept separately (sfk_op_get_smoother_schedule), and for the
+ * per-q-metric SIPG kernel the fused Chebyshev step over cells_per_block and
+ * the three orders, kept the same way.  Requires the
  * operator's data (metric, coefficient, velocity) to be set.
  * Tuning cache: when the environment variable SFK_TUNE_CACHE names a file,
  * an operator whose key (device architecture and CU count, kind, degree,
